@@ -1,0 +1,12 @@
+"""MI355X-native CESE Euler march with solvcon's solver API.
+
+The solver classes keep the Python surface of solvcon (Euler1DSolver,
+ShockTube, StaticMesh, EulerCore, GradientElement, EulerField) while every
+march, mesh build and reduction runs as a hand-written HIP kernel of
+libsolvcon_amd.so on the GPU.
+"""
+
+from . import onedim  # noqa: F401
+from ._lib import lib as _load_lib  # noqa: F401
+
+__all__ = ['onedim']

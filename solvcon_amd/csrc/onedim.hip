@@ -1,0 +1,786 @@
+/*
+ * One-dimensional Euler CESE march on the GPU: the device counterpart of
+ * solvcon's Euler1DCore (cpp/solvcon/onedim/Euler1DCore.hpp:39-349 and
+ * Euler1DCore.cpp:22-205).
+ *
+ * State is FP64 in HBM: coord[n], cfl[n], so0[n][3], so1[n][3], gamma[n].
+ * Every kernel evaluates the reference arithmetic in the reference order
+ * (built with -ffp-contract=off), so results match the CPU march bit for bit.
+ *
+ * Two march paths:
+ *  - n <= LDS_MAX_N: one persistent workgroup keeps so0/so1 in LDS and runs
+ *    every half step with workgroup barriers in between (the Sod-tube size,
+ *    2001 points, is launch-latency bound, so one launch per march call).
+ *  - larger n: one fused kernel per half step (so0, cfl and so1 together;
+ *    each thread owns one new solution point) plus a one-thread boundary
+ *    kernel between the two half steps.
+ */
+
+#include "common.hpp"
+
+#include <cmath>
+#include <memory>
+
+namespace scamd
+{
+namespace
+{
+
+constexpr double TINY1D = 1.e-100; // Euler1DCore.hpp:47,182
+constexpr double RGAS = 8.31446261815324; // Euler1DCore.hpp:48
+constexpr int LDS_MAX_N = 3400; // 48 B per point of so0+so1 in 160 KB of LDS
+constexpr unsigned BLOCK = 256;
+constexpr unsigned PBLOCK = 1024;
+
+// One solution element: the state of Euler1DKernel (Euler1DCore.hpp:180-282).
+struct Elem
+{
+    double x, xneg, xpos, xctr;
+    double u[3], ux[3], f[3], ut[3], ft[3], up[3];
+};
+
+// set_value + derive (Euler1DCore.hpp:194-244).
+__device__ inline void derive(Elem & e, double ga, double hdt, double x, double xneg, double xpos,
+                              double const u[3], double const ux[3])
+{
+    constexpr double t = TINY1D;
+    e.x = x;
+    e.xneg = xneg;
+    e.xpos = xpos;
+    e.xctr = (xpos + xneg) * 0.5;
+    double jac[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        e.u[k] = u[k];
+        e.ux[k] = ux[k];
+    }
+    double const u0 = u[0];
+    double const u1 = u[1];
+    double const u2 = u[2];
+    jac[0][0] = 0.0;
+    jac[0][1] = 1.0;
+    jac[0][2] = 0.0;
+    jac[1][0] = (ga - 3.0) / 2.0 * u1 * u1 / (u0 * u0 + t);
+    jac[1][1] = -(ga - 3.0) * u1 / (u0 + t);
+    jac[1][2] = ga - 1.0;
+    jac[2][0] = (ga - 1.0) * u1 * u1 * u1 / (u0 * u0 * u0 + t) - ga * u1 * u2 / (u0 * u0 + t);
+    jac[2][1] = ga * u2 / (u0 + t) - 3.0 / 2.0 * (ga - 1.0) * u1 * u1 / (u0 * u0 + t);
+    jac[2][2] = ga * u1 / (u0 + t);
+
+    e.f[0] = u1;
+    e.f[1] = (ga - 1.0) * u2 + (3.0 - ga) / 2.0 * u1 * u1 / (u0 + t);
+    e.f[2] = ga * u1 * u2 / (u0 + t) - (ga - 1.0) / 2.0 * u1 * u1 * u1 / (u0 * u0 + t);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        e.ut[i] = -jac[i][0] * ux[0] - jac[i][1] * ux[1] - jac[i][2] * ux[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        e.ft[i] = jac[i][0] * e.ut[0] + jac[i][1] * e.ut[1] + jac[i][2] * e.ut[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        e.up[i] = u[i] + (x - e.xctr) * ux[i] + hdt * e.ut[i];
+    }
+}
+
+// calc_flux_ll (Euler1DCore.hpp:246-255) of the left element plus
+// calc_flux_lr (:257-266) of the right one, divided by the CE width
+// (Euler1DCore.cpp:119-128).
+__device__ inline void new_so0(Elem const & xn, Elem const & xp, double hdt, double qdt, double dx, double out[3])
+{
+    double const dll = xn.xpos - xn.x;
+    double const mll = 0.5 * (xn.x + xn.xpos) - xn.xctr;
+    double const cll = xn.x - xn.xctr;
+    double const dlr = xp.x - xp.xneg;
+    double const mlr = 0.5 * (xp.x + xp.xneg) - xp.xctr;
+    double const clr = xp.x - xp.xctr;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        double const fll = dll * (xn.u[i] + mll * xn.ux[i]) + hdt * (xn.f[i] - (cll * xn.ut[i]) + (qdt * xn.ft[i]));
+        double const flr = dlr * (xp.u[i] + mlr * xp.ux[i]) - hdt * (xp.f[i] - (clr * xp.ut[i]) + (qdt * xp.ft[i]));
+        out[i] = (fll + flr) / dx;
+    }
+}
+
+// march_half_so1_alpha gradient (Euler1DCore.hpp:310-318).
+template <int ALPHA>
+__device__ inline double alpha_pow(double v)
+{
+    if constexpr (ALPHA == 1)
+    {
+        return v;
+    }
+    else
+    {
+        return v * v;
+    }
+}
+
+template <int ALPHA>
+__device__ inline void new_so1(Elem const & xn, Elem const & xp, double const utp[3], double out[3])
+{
+#pragma unroll
+    for (int iv = 0; iv < 3; ++iv)
+    {
+        double const duxn = (utp[iv] - xn.up[iv]) / (xn.xpos - xn.x);
+        double const duxp = (xp.up[iv] - utp[iv]) / (xp.x - xp.xneg);
+        double const fan = alpha_pow<ALPHA>(fabs(duxn));
+        double const fap = alpha_pow<ALPHA>(fabs(duxp));
+        out[iv] = (fap * duxn + fan * duxp) / (fap + fan + TINY1D);
+    }
+}
+
+// update_cfl body (Euler1DCore.cpp:76-93).
+__device__ inline double cfl_at(double ga, double hdt, double s0, double s1, double s2, double xm, double x, double xp)
+{
+    double wspd = s1;
+    wspd *= wspd;
+    double const ke = wspd / (2.0 * s0);
+    double pr = (ga - 1.0) * (s2 - ke);
+    pr = (pr + fabs(pr)) / 2.0;
+    wspd = sqrt(ga * pr / s0) + sqrt(wspd) / s0;
+    double const dxpos = xp - x;
+    double const dxneg = x - xm;
+    return hdt * wspd / (dxpos < dxneg ? dxpos : dxneg);
+}
+
+struct Geom1D
+{
+    int64_t start;
+    int64_t nout;
+};
+
+// Loop bounds of every half-step pass (Euler1DCore.cpp:73-76,99-100).
+__host__ __device__ inline Geom1D plane(int64_t n, bool odd)
+{
+    int64_t const start = 2 - (odd ? 1 : 0);
+    int64_t const stop = n - 2 - (odd ? 0 : 1);
+    int64_t const nout = stop > start ? (stop - start + 1) / 2 : 0;
+    return Geom1D{start, nout};
+}
+
+__device__ inline void load_elem(Elem & e, int64_t ic, double ga, double hdt, double const * coord, double const * so0,
+                                 double const * so1)
+{
+    double u[3] = {so0[ic * 3 + 0], so0[ic * 3 + 1], so0[ic * 3 + 2]};
+    double ux[3] = {so1[ic * 3 + 0], so1[ic * 3 + 1], so1[ic * 3 + 2]};
+    derive(e, ga, hdt, coord[ic], coord[ic - 1], coord[ic + 1], u, ux);
+}
+
+__global__ void k_cfl(int64_t start, int64_t nout, double hdt, double const * coord, double const * so0,
+                      double const * gamma, double * cfl)
+{
+    int64_t const k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nout)
+    {
+        return;
+    }
+    int64_t const it = start + 2 * k;
+    cfl[it] = cfl_at(gamma[it], hdt, so0[it * 3], so0[it * 3 + 1], so0[it * 3 + 2], coord[it - 1], coord[it],
+                     coord[it + 1]);
+}
+
+__global__ void k_so0(int64_t start, int64_t nout, double hdt, double qdt, double const * coord, double * so0,
+                      double const * so1, double const * gamma)
+{
+    int64_t const k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nout)
+    {
+        return;
+    }
+    int64_t const ic = start + 2 * k;
+    Elem xn;
+    Elem xp;
+    load_elem(xn, ic, gamma[ic], hdt, coord, so0, so1);
+    load_elem(xp, ic + 2, gamma[ic + 2], hdt, coord, so0, so1);
+    double out[3];
+    new_so0(xn, xp, hdt, qdt, coord[ic + 2] - coord[ic], out);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        so0[(ic + 1) * 3 + i] = out[i];
+    }
+}
+
+template <int ALPHA>
+__global__ void k_so1(int64_t start, int64_t nout, double hdt, double const * coord, double const * so0, double * so1,
+                      double const * gamma)
+{
+    int64_t const k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nout)
+    {
+        return;
+    }
+    int64_t const ic = start + 2 * k;
+    Elem xn;
+    Elem xp;
+    load_elem(xn, ic, gamma[ic], hdt, coord, so0, so1);
+    load_elem(xp, ic + 2, gamma[ic + 2], hdt, coord, so0, so1);
+    double const utp[3] = {so0[(ic + 1) * 3], so0[(ic + 1) * 3 + 1], so0[(ic + 1) * 3 + 2]};
+    double out[3];
+    new_so1<ALPHA>(xn, xp, utp, out);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        so1[(ic + 1) * 3 + i] = out[i];
+    }
+}
+
+// treat_boundary_so0/so1 (Euler1DCore.cpp:132-172).
+__global__ void k_bnd(int64_t n, double * a, double * b)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+    {
+        return;
+    }
+    for (int i = 0; i < 3; ++i)
+    {
+        a[3 + i] = a[6 + i];
+        a[(n - 2) * 3 + i] = a[(n - 3) * 3 + i];
+        if (b != nullptr)
+        {
+            b[3 + i] = b[6 + i];
+            b[(n - 2) * 3 + i] = b[(n - 3) * 3 + i];
+        }
+    }
+}
+
+// One fused half step (march_half1/2_alpha, Euler1DCore.hpp:322-337): the
+// new so0 point, the CFL of the input point and the new so1 point.
+template <int ALPHA>
+__global__ void k_half(int64_t start, int64_t nout, double hdt, double qdt, double const * coord, double * so0,
+                       double * so1, double const * gamma, double * cfl)
+{
+    int64_t const k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nout)
+    {
+        return;
+    }
+    int64_t const ic = start + 2 * k;
+    Elem xn;
+    Elem xp;
+    double const gn = gamma[ic];
+    load_elem(xn, ic, gn, hdt, coord, so0, so1);
+    load_elem(xp, ic + 2, gamma[ic + 2], hdt, coord, so0, so1);
+    double s0[3];
+    new_so0(xn, xp, hdt, qdt, coord[ic + 2] - coord[ic], s0);
+    cfl[ic] = cfl_at(gn, hdt, xn.u[0], xn.u[1], xn.u[2], xn.xneg, xn.x, xn.xpos);
+    double s1[3];
+    new_so1<ALPHA>(xn, xp, s0, s1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        so0[(ic + 1) * 3 + i] = s0[i];
+        so1[(ic + 1) * 3 + i] = s1[i];
+    }
+}
+
+// Persistent march for n <= LDS_MAX_N: one workgroup, so0 and so1 held in
+// LDS across all steps (march_alpha, Euler1DCore.hpp:339-349).
+template <int ALPHA>
+__global__ void __launch_bounds__(PBLOCK) k_march_lds(int64_t n, int64_t steps, double hdt, double qdt,
+                                                      double const * coord, double * gso0, double * gso1,
+                                                      double const * gamma, double * cfl)
+{
+    __shared__ double s0[LDS_MAX_N * 3];
+    __shared__ double s1[LDS_MAX_N * 3];
+    int const tid = threadIdx.x;
+    for (int64_t i = tid; i < n * 3; i += blockDim.x)
+    {
+        s0[i] = gso0[i];
+        s1[i] = gso1[i];
+    }
+    __syncthreads();
+    Geom1D const pe = plane(n, false);
+    Geom1D const po = plane(n, true);
+    for (int64_t it = 0; it < steps; ++it)
+    {
+        for (int half = 0; half < 2; ++half)
+        {
+            Geom1D const p = half == 0 ? pe : po;
+            for (int64_t k = tid; k < p.nout; k += blockDim.x)
+            {
+                int64_t const ic = p.start + 2 * k;
+                Elem xn;
+                Elem xp;
+                double const gn = gamma[ic];
+                {
+                    double u[3] = {s0[ic * 3], s0[ic * 3 + 1], s0[ic * 3 + 2]};
+                    double ux[3] = {s1[ic * 3], s1[ic * 3 + 1], s1[ic * 3 + 2]};
+                    derive(xn, gn, hdt, coord[ic], coord[ic - 1], coord[ic + 1], u, ux);
+                }
+                {
+                    int64_t const jc = ic + 2;
+                    double u[3] = {s0[jc * 3], s0[jc * 3 + 1], s0[jc * 3 + 2]};
+                    double ux[3] = {s1[jc * 3], s1[jc * 3 + 1], s1[jc * 3 + 2]};
+                    derive(xp, gamma[jc], hdt, coord[jc], coord[jc - 1], coord[jc + 1], u, ux);
+                }
+                double n0[3];
+                new_so0(xn, xp, hdt, qdt, coord[ic + 2] - coord[ic], n0);
+                if (it + 1 == steps)
+                {
+                    cfl[ic] = cfl_at(gn, hdt, xn.u[0], xn.u[1], xn.u[2], xn.xneg, xn.x, xn.xpos);
+                }
+                double n1[3];
+                new_so1<ALPHA>(xn, xp, n0, n1);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    s0[(ic + 1) * 3 + i] = n0[i];
+                    s1[(ic + 1) * 3 + i] = n1[i];
+                }
+            }
+            __syncthreads();
+            if (half == 0)
+            {
+                if (tid < 3)
+                {
+                    s0[3 + tid] = s0[6 + tid];
+                    s1[3 + tid] = s1[6 + tid];
+                }
+                else if (tid >= 32 && tid < 35)
+                {
+                    int const i = tid - 32;
+                    s0[(n - 2) * 3 + i] = s0[(n - 3) * 3 + i];
+                    s1[(n - 2) * 3 + i] = s1[(n - 3) * 3 + i];
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int64_t i = tid; i < n * 3; i += blockDim.x)
+    {
+        gso0[i] = s0[i];
+        gso1[i] = s1[i];
+    }
+}
+
+// Derived fields (Euler1DCore.hpp:106-168, Euler1DCore.cpp:37-68,174-205).
+__global__ void k_derived(int which, int64_t n, double const * so0, double const * gamma, double * out)
+{
+    int64_t const it = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (it >= n)
+    {
+        return;
+    }
+    double const u0 = so0[it * 3];
+    double const u1 = so0[it * 3 + 1];
+    double const u2 = so0[it * 3 + 2];
+    double const ga = gamma[it];
+    double pr = u1;
+    pr *= pr;
+    pr /= 2.0 * u0 + TINY1D;
+    pr = u2 - pr;
+    pr *= ga - 1.0;
+    double v = 0.0;
+    switch (which)
+    {
+    case 0: v = u0; break;
+    case 1: v = u1 / (u0 + TINY1D); break;
+    case 2: v = pr; break;
+    case 3:
+        v = (ga - 1.0) / RGAS;
+        v *= u2 / u0 - 0.5 * (u1 * u1) / (u0 * u0);
+        break;
+    case 4:
+        v = u2 / u0;
+        v -= 0.5 * ((u1 * u1) / (u0 * u0));
+        break;
+    default: v = pr / pow(u0, ga); break;
+    }
+    out[it] = v;
+}
+
+} /* end namespace */
+
+struct Core1D
+{
+    int64_t n = 0;
+    double dt = 0.0;
+    hipStream_t stream = nullptr;
+    DevBuf<double> coord, cfl, so0, so1, gamma;
+
+    Core1D(int64_t ncoord, double time_increment)
+        : n(ncoord)
+        , dt(time_increment)
+    {
+        SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        coord.alloc(n);
+        cfl.alloc(n);
+        so0.alloc(n * 3);
+        so1.alloc(n * 3);
+        gamma.alloc(n);
+        // Euler1DCore.cpp:30-34: zero-filled tables, gamma = 1.4.
+        SC_HIP(hipMemsetAsync(coord.ptr, 0, coord.bytes(), stream));
+        SC_HIP(hipMemsetAsync(cfl.ptr, 0, cfl.bytes(), stream));
+        SC_HIP(hipMemsetAsync(so0.ptr, 0, so0.bytes(), stream));
+        SC_HIP(hipMemsetAsync(so1.ptr, 0, so1.bytes(), stream));
+        fill_gamma(1.4);
+    }
+
+    ~Core1D()
+    {
+        if (stream != nullptr)
+        {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void fill_gamma(double g);
+
+    DevBuf<double> & buf(int which)
+    {
+        switch (which)
+        {
+        case 0: return coord;
+        case 1: return cfl;
+        case 2: return so0;
+        case 3: return so1;
+        case 4: return gamma;
+        default: fail(SC_EVALUE, "Euler1DCore: unknown array id");
+        }
+    }
+
+    double hdt() const { return dt / 2.0; }
+    double qdt() const { return hdt() / 2.0; }
+
+    void sync() { SC_HIP(hipStreamSynchronize(stream)); }
+
+    void update_cfl(bool odd)
+    {
+        Geom1D const p = plane(n, odd);
+        if (p.nout > 0)
+        {
+            k_cfl<<<grid_for(p.nout, BLOCK), BLOCK, 0, stream>>>(p.start, p.nout, hdt(), coord.ptr, so0.ptr,
+                                                                 gamma.ptr, cfl.ptr);
+            SC_HIP(hipGetLastError());
+        }
+    }
+
+    void half_so0(bool odd)
+    {
+        Geom1D const p = plane(n, odd);
+        if (p.nout > 0)
+        {
+            k_so0<<<grid_for(p.nout, BLOCK), BLOCK, 0, stream>>>(p.start, p.nout, hdt(), qdt(), coord.ptr, so0.ptr,
+                                                                 so1.ptr, gamma.ptr);
+            SC_HIP(hipGetLastError());
+        }
+    }
+
+    void half_so1(int alpha, bool odd)
+    {
+        Geom1D const p = plane(n, odd);
+        if (p.nout <= 0)
+        {
+            return;
+        }
+        if (alpha == 1)
+        {
+            k_so1<1><<<grid_for(p.nout, BLOCK), BLOCK, 0, stream>>>(p.start, p.nout, hdt(), coord.ptr, so0.ptr,
+                                                                    so1.ptr, gamma.ptr);
+        }
+        else
+        {
+            k_so1<2><<<grid_for(p.nout, BLOCK), BLOCK, 0, stream>>>(p.start, p.nout, hdt(), coord.ptr, so0.ptr,
+                                                                    so1.ptr, gamma.ptr);
+        }
+        SC_HIP(hipGetLastError());
+    }
+
+    void boundary(bool do0, bool do1)
+    {
+        k_bnd<<<1, 64, 0, stream>>>(n, do0 ? so0.ptr : so1.ptr, (do0 && do1) ? so1.ptr : nullptr);
+        SC_HIP(hipGetLastError());
+    }
+
+    void half(int alpha, bool odd)
+    {
+        Geom1D const p = plane(n, odd);
+        if (p.nout <= 0)
+        {
+            return;
+        }
+        if (alpha == 1)
+        {
+            k_half<1><<<grid_for(p.nout, BLOCK), BLOCK, 0, stream>>>(p.start, p.nout, hdt(), qdt(), coord.ptr,
+                                                                     so0.ptr, so1.ptr, gamma.ptr, cfl.ptr);
+        }
+        else
+        {
+            k_half<2><<<grid_for(p.nout, BLOCK), BLOCK, 0, stream>>>(p.start, p.nout, hdt(), qdt(), coord.ptr,
+                                                                     so0.ptr, so1.ptr, gamma.ptr, cfl.ptr);
+        }
+        SC_HIP(hipGetLastError());
+    }
+
+    void march(int alpha, int64_t steps)
+    {
+        if (steps <= 0)
+        {
+            return;
+        }
+        if (n <= LDS_MAX_N)
+        {
+            // The persistent kernel writes the CFL of the last step only: the
+            // reference overwrites cfl every half step, so only the last
+            // step's values survive either way.
+            if (alpha == 1)
+            {
+                k_march_lds<1><<<1, PBLOCK, 0, stream>>>(n, steps, hdt(), qdt(), coord.ptr, so0.ptr, so1.ptr,
+                                                         gamma.ptr, cfl.ptr);
+            }
+            else
+            {
+                k_march_lds<2><<<1, PBLOCK, 0, stream>>>(n, steps, hdt(), qdt(), coord.ptr, so0.ptr, so1.ptr,
+                                                         gamma.ptr, cfl.ptr);
+            }
+            SC_HIP(hipGetLastError());
+            return;
+        }
+        for (int64_t it = 0; it < steps; ++it)
+        {
+            half(alpha, false);
+            boundary(true, true);
+            half(alpha, true);
+        }
+    }
+
+    void derived(int which, double * dout)
+    {
+        k_derived<<<grid_for(n, BLOCK), BLOCK, 0, stream>>>(which, n, so0.ptr, gamma.ptr, dout);
+        SC_HIP(hipGetLastError());
+    }
+};
+
+namespace
+{
+__global__ void k_fill(int64_t n, double * a, double v)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        a[i] = v;
+    }
+}
+} /* end namespace */
+
+void Core1D::fill_gamma(double g)
+{
+    k_fill<<<grid_for(n, BLOCK), BLOCK, 0, stream>>>(n, gamma.ptr, g);
+    SC_HIP(hipGetLastError());
+}
+
+} /* end namespace scamd */
+
+using scamd::Core1D;
+using scamd::guard;
+
+extern "C"
+{
+
+int sc1d_create(int64_t ncoord, double time_increment, void ** out)
+{
+    return guard(
+        [&]()
+        {
+            // Euler1DCore.cpp:25-28
+            if (ncoord % 2 == 0)
+            {
+                scamd::fail(scamd::SC_EVALUE, "ncoord cannot be even");
+            }
+            if (ncoord < 5)
+            {
+                scamd::fail(scamd::SC_EVALUE, "ncoord must be at least 5");
+            }
+            auto * h = new Core1D(ncoord, time_increment);
+            h->sync();
+            *out = h;
+        });
+}
+
+int sc1d_clone(void * h, void ** out)
+{
+    return guard(
+        [&]()
+        {
+            auto & a = *static_cast<Core1D *>(h);
+            auto * b = new Core1D(a.n, a.dt);
+            for (int w = 0; w < 5; ++w)
+            {
+                SC_HIP(hipMemcpyAsync(b->buf(w).ptr, a.buf(w).ptr, a.buf(w).bytes(), hipMemcpyDeviceToDevice,
+                                      a.stream));
+            }
+            a.sync();
+            b->sync();
+            *out = b;
+        });
+}
+
+void sc1d_destroy(void * h) { delete static_cast<Core1D *>(h); }
+
+int sc1d_info(void * h, int64_t * ncoord, double * time_increment)
+{
+    auto & a = *static_cast<Core1D *>(h);
+    *ncoord = a.n;
+    *time_increment = a.dt;
+    return 0;
+}
+
+int sc1d_device_ptr(void * h, int which, double ** ptr, int64_t * count)
+{
+    return guard(
+        [&]()
+        {
+            auto & b = static_cast<Core1D *>(h)->buf(which);
+            *ptr = b.ptr;
+            *count = b.count;
+        });
+}
+
+int sc1d_upload(void * h, int which, double const * host, int64_t count)
+{
+    return guard(
+        [&]()
+        {
+            auto & a = *static_cast<Core1D *>(h);
+            auto & b = a.buf(which);
+            if (count != b.count)
+            {
+                scamd::fail(scamd::SC_EVALUE, "Euler1DCore: array size mismatch");
+            }
+            SC_HIP(hipMemcpyAsync(b.ptr, host, b.bytes(), hipMemcpyHostToDevice, a.stream));
+            a.sync();
+        });
+}
+
+int sc1d_download(void * h, int which, double * host, int64_t count)
+{
+    return guard(
+        [&]()
+        {
+            auto & a = *static_cast<Core1D *>(h);
+            auto & b = a.buf(which);
+            if (count != b.count)
+            {
+                scamd::fail(scamd::SC_EVALUE, "Euler1DCore: array size mismatch");
+            }
+            SC_HIP(hipMemcpyAsync(host, b.ptr, b.bytes(), hipMemcpyDeviceToHost, a.stream));
+            a.sync();
+        });
+}
+
+int sc1d_update_cfl(void * h, int odd_plane)
+{
+    return guard([&]() { auto & a = *static_cast<Core1D *>(h); a.update_cfl(odd_plane != 0); a.sync(); });
+}
+
+int sc1d_march_half_so0(void * h, int odd_plane)
+{
+    return guard([&]() { auto & a = *static_cast<Core1D *>(h); a.half_so0(odd_plane != 0); a.sync(); });
+}
+
+int sc1d_march_half_so1(void * h, int alpha, int odd_plane)
+{
+    return guard(
+        [&]()
+        {
+            if (alpha != 1 && alpha != 2)
+            {
+                scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
+            }
+            auto & a = *static_cast<Core1D *>(h);
+            a.half_so1(alpha, odd_plane != 0);
+            a.sync();
+        });
+}
+
+int sc1d_treat_boundary_so0(void * h)
+{
+    return guard([&]() { auto & a = *static_cast<Core1D *>(h); a.boundary(true, false); a.sync(); });
+}
+
+int sc1d_treat_boundary_so1(void * h)
+{
+    return guard([&]() { auto & a = *static_cast<Core1D *>(h); a.boundary(false, true); a.sync(); });
+}
+
+int sc1d_setup_march(void * h)
+{
+    return guard([&]() { auto & a = *static_cast<Core1D *>(h); a.update_cfl(false); a.sync(); });
+}
+
+int sc1d_march_half(void * h, int alpha, int second)
+{
+    return guard(
+        [&]()
+        {
+            if (alpha != 1 && alpha != 2)
+            {
+                scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
+            }
+            auto & a = *static_cast<Core1D *>(h);
+            a.half(alpha, second != 0);
+            a.sync();
+        });
+}
+
+int sc1d_march(void * h, int alpha, int64_t steps)
+{
+    return guard(
+        [&]()
+        {
+            if (alpha != 1 && alpha != 2)
+            {
+                scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
+            }
+            auto & a = *static_cast<Core1D *>(h);
+            a.march(alpha, steps);
+            a.sync();
+        });
+}
+
+int sc1d_march_async(void * h, int alpha, int64_t steps)
+{
+    return guard(
+        [&]()
+        {
+            if (alpha != 1 && alpha != 2)
+            {
+                scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
+            }
+            static_cast<Core1D *>(h)->march(alpha, steps);
+        });
+}
+
+int sc1d_sync(void * h)
+{
+    return guard([&]() { static_cast<Core1D *>(h)->sync(); });
+}
+
+int sc1d_derived(void * h, int which, double * host_out)
+{
+    return guard(
+        [&]()
+        {
+            auto & a = *static_cast<Core1D *>(h);
+            if (which < 0 || which > 5)
+            {
+                scamd::fail(scamd::SC_EVALUE, "unknown derived field");
+            }
+            scamd::DevBuf<double> tmp;
+            tmp.alloc(a.n);
+            a.derived(which, tmp.ptr);
+            SC_HIP(hipMemcpyAsync(host_out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, a.stream));
+            a.sync();
+        });
+}
+
+} /* end extern "C" */

@@ -1,0 +1,463 @@
+"""Pin the oracle's EulerCore restatement.
+
+Pinned by: the reference's hand-computed CE volumes
+(tests/test_euler_multidim.py:74-120), the GradientElement linear-algebra
+goldens (gtests/test_nopython_multidim.cpp:82-135), closed forms (flux
+homogeneity, free stream, linear-field recovery, BC reflections, the
+uniform-field CFL) and independent numpy restatements that use an analytic
+flux with a finite-difference Jacobian (the reference tests' own strategy,
+tests/test_multidim.py:9-149).
+"""
+
+import numpy as np
+import pytest
+from numpy.testing import assert_almost_equal
+
+import meshdefs as MD
+
+GAMMA, RHO, PRES = 1.4, 1.2, 0.9
+VEL = (0.3, -0.15, 0.05)
+ALL = ['triangles', 'quad', 'mixed', 'tet', 'hex', 'prism', 'pyramid']
+
+
+def euler_flux(u, gamma):
+    """Analytic Euler flux f[ieq][d] of a conserved row u."""
+    nd = u.size - 2
+    rho, mom, en = u[0], u[1:1 + nd], u[nd + 1]
+    vel = mom / rho
+    p = (gamma - 1.0) * (en - 0.5 * rho * (vel @ vel))
+    f = np.zeros((u.size, nd))
+    f[0] = mom
+    for k in range(nd):
+        f[1 + k] = mom[k] * vel
+        f[1 + k, k] += p
+    f[nd + 1] = (en + p) * vel
+    return f
+
+
+def jac_fd(u, gamma, h=1e-6):
+    neq = u.size
+    jac = np.zeros((neq, neq, neq - 2))
+    for j in range(neq):
+        e = np.zeros(neq)
+        e[j] = h
+        jac[:, j, :] = (euler_flux(u + e, gamma) - euler_flux(u - e, gamma)) \
+            / (2 * h)
+    return jac
+
+
+def build(orc, name, dt=0.01):
+    m = orc.make_mesh(*MD.SINGLES[name])
+    return m, orc.Euler(m, dt)
+
+
+def rows(arr, ngh, icl):
+    return arr[ngh + icl]
+
+
+def centre(mesh, ec, cell):
+    """Neighbour solution point: CE centroid for body, mesh centroid for
+    ghost cells."""
+    nd = mesh.ndim
+    if cell >= 0:
+        return ec.cecnd[ec.ngstcell + cell, :nd].copy()
+    return mesh.clcnd[mesh.ngstcell + cell].copy()
+
+
+def test_ce_triangle_hand_values(orc):
+    m, ec = build(orc, 'triangles')
+    g = ec.ngstcell
+    assert_almost_equal(ec.cevol[g + 0, 1:4], [0.5, 2 / 3, 0.5], decimal=12)
+    assert_almost_equal(ec.cevol[g + 0, 0], 0.5 + 2 / 3 + 0.5, decimal=12)
+
+
+def test_ce_quad_hand_values(orc):
+    m, ec = build(orc, 'quad')
+    g = ec.ngstcell
+    assert_almost_equal(ec.cevol[g, :5], [2.0, 0.5, 0.5, 0.5, 0.5],
+                        decimal=12)
+    sf = ec.sfnml[0].reshape(6, 4, 2)
+    mag = np.linalg.norm(sf, axis=2)
+    assert_almost_equal(mag[mag > 1e-14], 0.5 * np.sqrt(2.0), decimal=12)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_bce_sum_and_total(orc, name):
+    m, ec = build(orc, name)
+    g = ec.ngstcell
+    nd = m.ndim
+    for icl in range(ec.ncell):
+        nfc = m.clfcs[m.ngstcell + icl, 0]
+        assert_almost_equal(ec.cevol[g + icl, 1:1 + nfc].sum(),
+                            ec.cevol[g + icl, 0], decimal=12)
+    # Ghosts mirror the body cells, so the CCEs tile twice the body volume.
+    assert_almost_equal(ec.cevol[g:, 0].sum(),
+                        2 * m.clvol[m.ngstcell:].sum(), decimal=12)
+    # The CCE centroid is the volume-weighted mean of its BCE centroids.
+    for icl in range(ec.ncell):
+        nfc = m.clfcs[m.ngstcell + icl, 0]
+        v = ec.cevol[g + icl, 1:1 + nfc]
+        c = ec.cecnd[g + icl, nd:nd * (nfc + 1)].reshape(nfc, nd)
+        assert_almost_equal((v[:, None] * c).sum(0) / v.sum(),
+                            ec.cecnd[g + icl, :nd], decimal=12)
+
+
+def test_ge_linalg_goldens(orc):
+    a2 = np.array([[2, 1], [1, 3]], dtype=float)
+    assert orc.ge_det(2, a2) == 5.0
+    assert orc.ge_adj(2, a2).tolist() == [[3, -1], [-1, 2]]
+    a3 = np.array([[1, 2, 3], [0, 1, 4], [5, 6, 0]], dtype=float)
+    assert orc.ge_det(3, a3) == 1.0
+    assert orc.ge_adj(3, a3).tolist() == [[-24, 18, 5], [20, -15, -4],
+                                          [-5, 4, 1]]
+    x = orc.ge_solve(3, a3, [6, -1, 2])
+    assert_almost_equal(a3 @ x, [6, -1, 2], decimal=12)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_gradient_element(orc, name):
+    m, ec = build(orc, name)
+    nd = m.ndim
+    cecnd = ec.cecnd
+    g = np.array([1.5, -2.7, 0.9][:nd])
+    for icl in range(ec.ncell):
+        ge = orc.gradient_element(m, cecnd, ec.ngstcell, icl, 1.0)
+        assert ge['nfge_inverse'] == 1.0 / ge['nfge']
+        assert np.linalg.matrix_rank(ge['idis']) == nd
+        for ifl in range(ge['clnfc']):
+            jcl = ge['rcl'][ifl]
+            assert_almost_equal(ge['idis'][ifl] + cecnd[ec.ngstcell + icl, :nd],
+                                ge['jdis'][ifl] + centre(m, ec, jcl),
+                                decimal=12)
+        for ifge in range(ge['nfge']):
+            dst = ge['idis'][ge['faces'][ifge] - 1]
+            assert abs(np.linalg.det(dst)) > 1e-10
+            got = orc.ge_solve(nd, dst, dst @ g)
+            assert_almost_equal(got, g, decimal=9)
+        # tau = 0 places the evaluation points on the BCE centroids.
+        ge0 = orc.gradient_element(m, cecnd, ec.ngstcell, icl, 0.0)
+        row = cecnd[ec.ngstcell + icl]
+        shift = ge0['idis'][0] + row[:nd] - row[nd:2 * nd]
+        for ifl in range(1, ge0['clnfc']):
+            assert_almost_equal(ge0['idis'][ifl] + row[:nd],
+                                row[(ifl + 1) * nd:(ifl + 2) * nd] + shift,
+                                decimal=12)
+
+
+def uniform_row(nd):
+    v = np.array(VEL[:nd])
+    en = PRES / (GAMMA - 1.0) + 0.5 * RHO * (v @ v)
+    return np.array([RHO] + list(RHO * v) + [en])
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_init_and_cfl(orc, name):
+    m, ec = build(orc, name)
+    nd, g = m.ndim, ec.ngstcell
+    with pytest.raises(ValueError):
+        ec.init_solution(GAMMA, RHO, [0.1] * (nd - 1), PRES)
+    for bad in (dict(gamma=1.0), dict(rho=0.0), dict(p=-1.0)):
+        kw = dict(gamma=GAMMA, rho=RHO, v=list(VEL[:nd]), p=PRES)
+        kw.update(bad)
+        with pytest.raises(ValueError):
+            ec.init_solution(**kw)
+    ec.init_solution(GAMMA, RHO, list(VEL[:nd]), PRES)
+    row = uniform_row(nd)
+    assert_almost_equal(ec.so0n[g:], np.tile(row, (ec.ncell, 1)))
+    assert (ec.so0n[:g] == 0).all() and (ec.gamma == GAMMA).all()
+    ec.call('calc_cfl')
+    v = np.array(VEL[:nd])
+    wspd = np.sqrt(GAMMA * PRES / RHO) + np.sqrt(v @ v)
+    for icl in range(ec.ncell):
+        nfc = m.clfcs[m.ngstcell + icl, 0]
+        ce = ec.cecnd[g + icl]
+        dist = min(np.linalg.norm(ce[k * nd:(k + 1) * nd] - ce[:nd])
+                   for k in range(1, nfc + 1))
+        assert_almost_equal(ec.cflo[g + icl], 0.005 * wspd / dist, decimal=13)
+        assert_almost_equal(ec.cflc[g + icl], ec.cflo[g + icl], decimal=13)
+        assert_almost_equal(ec.so0n[g + icl, nd + 1], row[nd + 1])
+    # Negative pressure: clamped, cflc forced to 1, energy = kinetic part.
+    ec.so0n[g:, nd + 1] = 0.0
+    ec.call('calc_cfl')
+    ke = (RHO * v) @ (RHO * v) / (2 * RHO)
+    assert_almost_equal(ec.cflc[g:], 1.0)
+    assert_almost_equal(ec.so0n[g:, nd + 1], ke)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_update_swaps(orc, name):
+    m, ec = build(orc, name)
+    ec.init_solution(GAMMA, RHO, list(VEL[:m.ndim]), PRES)
+    ec.so1n[...] = np.arange(ec.so1n.size).reshape(ec.so1n.shape)
+    before = {k: getattr(ec, k).copy() for k in ('so0c', 'so0n', 'so1c',
+                                                  'so1n')}
+    ec.call('update')
+    assert (ec.so0c == before['so0n']).all()
+    assert (ec.so0n == before['so0c']).all()
+    assert (ec.so1c == before['so1n']).all()
+    assert (ec.so1n == before['so1c']).all()
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_solt_is_flux(orc, name):
+    # Degree-one homogeneity: J(u) . (c u) = c f(u), so so0t = -sum c_d f_d.
+    m, ec = build(orc, name)
+    nd, g = m.ndim, ec.ngstcell
+    row = uniform_row(nd)
+    coef = np.array([0.7, -1.3, 0.4][:nd])
+    ec.gamma[:] = GAMMA
+    ec.so0c[:] = row
+    ec.so1c[:] = row[None, :, None] * coef[None, None, :]
+    ec.call('calc_solt')
+    f = euler_flux(row, GAMMA)
+    assert_almost_equal(ec.so0t[g:], np.tile(-(f @ coef), (ec.ncell, 1)),
+                        decimal=12)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_soln_freestream(orc, name):
+    m, ec = build(orc, name)
+    row = uniform_row(m.ndim)
+    ec.gamma[:] = GAMMA
+    ec.so0c[:] = row
+    ec.so1c[:] = 0.0
+    ec.call('calc_solt')
+    ec.call('calc_soln')
+    assert_almost_equal(ec.so0n[ec.ngstcell:], np.tile(row, (ec.ncell, 1)),
+                        decimal=12)
+
+
+def _seed_nonuniform(m, ec):
+    nd, neq = m.ndim, m.ndim + 2
+    base = [1.2, 0.3, -0.15, 0.2, 2.5]
+    for cell in range(-ec.ngstcell, ec.ncell):
+        x = centre(m, ec, cell)
+        r = ec.ngstcell + cell
+        ec.gamma[r] = GAMMA
+        for ieq in range(neq):
+            s = 1.0 + 0.1 * x[0] + 0.03 * ieq + 0.07 * x[1:].sum()
+            ec.so0c[r, ieq] = base[ieq] * s
+            ec.so0t[r, ieq] = 0.01 * (ieq + 1)
+            for d in range(nd):
+                ec.so1c[r, ieq, d] = 0.02 * (ieq + 1) * (d + 1)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_soln_against_fd_restatement(orc, name):
+    m, ec = build(orc, name)
+    _seed_nonuniform(m, ec)
+    ec.call('calc_soln')
+    nd, neq, g = m.ndim, m.ndim + 2, ec.ngstcell
+    dt = ec.time_increment
+    for icl in range(ec.ncell):
+        acc = np.zeros(neq)
+        mr = m.ngstcell + icl
+        for ifl in range(1, m.clfcs[mr, 0] + 1):
+            ifc = m.clfcs[mr, ifl]
+            fr = m.ngstface + ifc
+            jcl = m.fccls[fr, 0] + m.fccls[fr, 1] - icl
+            jce = centre(m, ec, jcl)
+            bc = ec.cecnd[g + icl, ifl * nd:(ifl + 1) * nd]
+            js = ec.so0c[g + jcl]
+            j1 = ec.so1c[g + jcl]
+            jt = ec.so0t[g + jcl]
+            acc += (js + j1 @ (bc - jce)) * ec.cevol[g + icl, ifl]
+            f = euler_flux(js, GAMMA)
+            jac = jac_fd(js, GAMMA)
+            for inf in range(m.fcnds[fr, 0]):
+                si = (ifl - 1) * 4 + inf
+                us = 0.25 * dt * jt + j1 @ (ec.sfcnd[icl, si] - jce)
+                df = f + np.einsum('ijd,j->id', jac, us)
+                acc -= 0.5 * dt * (df @ ec.sfnml[icl, si])
+        assert_almost_equal(ec.so0n[g + icl], acc / ec.cevol[g + icl, 0],
+                            decimal=9)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_dsoln_linear_field(orc, name):
+    m, ec = build(orc, name)
+    nd, neq, g = m.ndim, m.ndim + 2, ec.ngstcell
+    gr = np.array([[0.1 * (i + 1) + 0.01 * (d + 1) for d in range(nd)]
+                   for i in range(neq)])
+    c = np.array([0.5 + 0.3 * i for i in range(neq)])
+    for cell in range(-g, ec.ncell):
+        val = c + gr @ centre(m, ec, cell)
+        ec.so0c[g + cell] = val
+        ec.so0n[g + cell] = val
+        ec.so0t[g + cell] = 0.0
+        ec.so1c[g + cell] = gr
+        ec.cflc[g + cell] = 0.5
+    ec.call('calc_dsoln')
+    for icl in range(ec.ncell):
+        assert_almost_equal(ec.so1n[g + icl], gr, decimal=9)
+
+
+@pytest.mark.parametrize('name', ALL)
+def test_dsoln_against_restatement(orc, name):
+    m, ec = build(orc, name)
+    nd, neq, g = m.ndim, m.ndim + 2, ec.ngstcell
+    for cell in range(-g, ec.ncell):
+        x = centre(m, ec, cell)
+        for ieq in range(neq):
+            val = 1.0 + 0.5 * x[0] + 0.4 * x[0] ** 2 + 0.2 * ieq \
+                + 0.3 * x[1:].sum()
+            ec.so0c[g + cell, ieq] = val
+            ec.so0n[g + cell, ieq] = val
+            ec.so0t[g + cell, ieq] = 0.0
+            for d in range(nd):
+                ec.so1c[g + cell, ieq, d] = 0.1 * (ieq + 1) + 0.05 * x[0] * (d + 1)
+        ec.cflc[g + cell] = 0.5
+    ec.call('calc_dsoln')
+    hdt = 0.5 * ec.time_increment
+    az = 1e-200
+    active = False
+    for icl in range(ec.ncell):
+        acfl = abs(ec.cflc[g + icl])
+        ge = orc.gradient_element(m, ec.cecnd, g, icl,
+                                  ec.taumin + acfl * ec.tauscale)
+        nfge, ofg1 = ge['nfge'], ge['nfge_inverse']
+        grad = np.zeros((nfge, neq, nd))
+        for ifge in range(nfge):
+            fl = ge['faces'][ifge] - 1
+            dst = ge['idis'][fl]
+            for ieq in range(neq):
+                udf = np.array([ec.so0c[g + ge['rcl'][k], ieq]
+                                + hdt * ec.so0t[g + ge['rcl'][k], ieq]
+                                - ec.so0n[g + icl, ieq]
+                                + ge['jdis'][k] @ ec.so1c[g + ge['rcl'][k], ieq]
+                                for k in fl])
+                grad[ifge, ieq] = np.linalg.solve(dst, udf)
+        w = 1.0 / np.sqrt((grad ** 2).sum(axis=2) + az)
+        w = w / w.sum(axis=0) - ofg1
+        if np.abs(w).max() > 1e-9:
+            active = True
+        sm = np.minimum(np.minimum((1 - ofg1) / (w.max(0).clip(0) + az),
+                                   -ofg1 / (w.min(0).clip(None, 0) - az)),
+                        ec.sigma0 / acfl)
+        expect = ((ofg1 + sm[None, :] * w)[:, :, None] * grad).sum(0)
+        assert_almost_equal(ec.so1n[g + icl], expect, decimal=9)
+    assert active or ec.ncell == 1 and name in ('quad', 'hex')
+
+
+BCS = ['triangles', 'quad', 'mixed', 'tet', 'hex']
+
+
+def _bnd(m):
+    ngf = m.ngstface
+    return [(int(f), int(m.fccls[ngf + f, 0]), int(m.fccls[ngf + f, 1]))
+            for f in m.bndfcs[:, 0]]
+
+
+@pytest.mark.parametrize('name', BCS)
+def test_bc_identities(orc, name):
+    m, ec = build(orc, name)
+    nd, neq, g = m.ndim, m.ndim + 2, ec.ngstcell
+    faces = [f for f, _, _ in _bnd(m)]
+    for ifc in faces:
+        mat = ec.normal_matrix(ifc)
+        assert_almost_equal(mat @ mat.T, np.eye(nd), decimal=12)
+        assert_almost_equal(mat[0], m.fcnml[m.ngstface + ifc], decimal=12)
+        assert_almost_equal(np.linalg.det(mat), 1.0, decimal=12)
+    # Non-reflective.
+    for icl in range(ec.ncell):
+        ec.so0n[g + icl] = 1.0 + 0.1 * icl + 0.37 * np.arange(neq)
+        for ieq in range(neq):
+            ec.so1c[g + icl, ieq] = 0.2 + 0.1 * ieq - 0.05 * np.arange(nd) \
+                + 0.03 * icl
+    ec.add_bc(orc.BC_NONREFL, faces)
+    ec.call('bc_soln')
+    ec.call('bc_dsoln')
+    for ifc, icl, jcl in _bnd(m):
+        n = m.fcnml[m.ngstface + ifc]
+        assert (ec.so0n[g + jcl] == ec.so0n[g + icl]).all()
+        gi, gg = ec.so1c[g + icl], ec.so1n[g + jcl]
+        assert_almost_equal(gg, gi - np.outer(gi @ n, n), decimal=12)
+    # Slip wall.
+    ec.call('clear_bc')
+    ec.add_bc(orc.BC_SLIPWALL, faces)
+    ec.call('bc_soln')
+    ec.call('bc_dsoln')
+    for ifc, icl, jcl in _bnd(m):
+        n = m.fcnml[m.ngstface + ifc]
+        R = np.eye(nd) - 2 * np.outer(n, n)
+        si, sg = ec.so0n[g + icl], ec.so0n[g + jcl]
+        assert sg[0] == si[0] and sg[-1] == si[-1]
+        assert_almost_equal(sg[1:1 + nd], R @ si[1:1 + nd], decimal=12)
+        gi, gg = ec.so1c[g + icl], ec.so1n[g + jcl]
+        for ieq in (0, neq - 1):
+            assert_almost_equal(gg[ieq], R @ gi[ieq], decimal=12)
+        assert_almost_equal(gg[1:1 + nd], R @ gi[1:1 + nd] @ R, decimal=12)
+    # Inlet.
+    ec.call('clear_bc')
+    val = [1.3] + [2.0, 0.3, -0.1][:nd] + [0.8, GAMMA]
+    with pytest.raises(ValueError):
+        ec.add_bc(orc.BC_INLET, faces, [1.0] * (nd + 2))
+    for bad in ([0.0] + [0.0] * nd + [1.0, 1.4],
+                [1.0] + [0.0] * nd + [-1.0, 1.4],
+                [1.0] + [0.0] * nd + [1.0, 1.0]):
+        with pytest.raises(ValueError):
+            ec.add_bc(orc.BC_INLET, faces, bad)
+    with pytest.raises(ValueError):
+        ec.add_bc(orc.BC_NONREFL, [10 ** 7])
+    ec.add_bc(orc.BC_INLET, faces, val)
+    ec.call('bc_soln')
+    ec.call('bc_dsoln')
+    rho, v, p, ga = val[0], np.array(val[1:1 + nd]), val[1 + nd], val[2 + nd]
+    for ifc, icl, jcl in _bnd(m):
+        assert_almost_equal(ec.so0n[g + jcl],
+                            [rho] + list(rho * v) + [p / (ga - 1) + 0.5 * rho * (v @ v)])
+        assert (ec.so1n[g + jcl] == 0).all()
+
+
+def test_channel_march_steady(orc):
+    # tests/test_multidim.py:1029-1084: supersonic inlet, slip walls,
+    # non-reflective outflow; a uniform stream is the steady state.
+    m = orc.make_mesh(*MD.quad_channel(4, 2, 4.0, 2.0))
+    ngf = m.ngstface
+    left, right, walls = [], [], []
+    for f in m.bndfcs[:, 0]:
+        cx = m.fccnd[ngf + f, 0]
+        (left if abs(cx) < 1e-9 else right if abs(cx - 4.0) < 1e-9
+         else walls).append(int(f))
+    ec = orc.Euler(m, 0.04)
+    ec.init_solution(1.4, 1.0, [2.0, 0.0], 1.0)
+    ec.add_bc(orc.BC_INLET, left, [1.0, 2.0, 0.0, 1.0, 1.4])
+    ec.add_bc(orc.BC_NONREFL, right)
+    ec.add_bc(orc.BC_SLIPWALL, walls)
+    ec.call('bc_soln')
+    ec.call('bc_dsoln')
+    ec.march(5)
+    g = ec.ngstcell
+    assert np.isfinite(ec.so0n).all()
+    assert_almost_equal(ec.so0n[g:, 0], 1.0, decimal=6)
+    for f in left:
+        assert_almost_equal(ec.so0n[g + m.fccls[ngf + f, 1], 0], 1.0)
+
+
+@pytest.mark.parametrize('name', ['triangles', 'mixed'])
+def test_march_bounded(orc, name):
+    m, ec = build(orc, name)
+    ec.init_solution(GAMMA, RHO, list(VEL[:m.ndim]), PRES)
+    ec.march(3)
+    assert np.isfinite(ec.so0n).all()
+    assert (ec.so0n[ec.ngstcell:, 0] > 0).all()
+    assert np.abs(ec.so0n).max() < 1e6
+
+
+def test_threads_bitwise(orc):
+    # The OpenMP mode matches the serial mode bit for bit.
+    d = MD.hex_block(4, 3, 3, jitter=0.2, seed=5)
+    outs = []
+    for nt in (1, 4):
+        m = orc.make_mesh(*d)
+        ec = orc.Euler(m, 0.002, nthreads=nt)
+        ec.init_solution(1.4, 1.0, [0.1, 0.2, 0.0], 1.0)
+        g = ec.ngstcell
+        ec.so0n[g:, 4] += np.linspace(0, 1, ec.ncell)
+        ec.add_bc(orc.BC_NONREFL, m.bndfcs[:, 0])
+        ec.call('bc_soln')
+        ec.call('bc_dsoln')
+        ec.march(3)
+        outs.append((ec.so0n.copy(), ec.so1n.copy()))
+    assert (outs[0][0] == outs[1][0]).all()
+    assert (outs[0][1] == outs[1][1]).all()
