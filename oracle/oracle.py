@@ -293,8 +293,11 @@ def make_mesh(ndim, ndcrd, cltpn, clnds, build=True, bcs=()):
     m.ndcrd[:, :] = ndcrd
     m.cltpn[:] = cltpn
     cl = m.clnds
-    for i, row in enumerate(clnds):
-        cl[i, :len(row)] = row
+    if isinstance(clnds, np.ndarray):
+        cl[:, :clnds.shape[1]] = clnds
+    else:
+        for i, row in enumerate(clnds):
+            cl[i, :len(row)] = row
     if build:
         m.build_interior(do_metric=True, build_edge=True)
         for name, faces in bcs:

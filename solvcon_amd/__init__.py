@@ -8,5 +8,8 @@ libsolvcon_amd.so on the GPU.
 
 from . import onedim  # noqa: F401
 from ._lib import lib as _load_lib  # noqa: F401
+from .mesh import StaticMesh, StaticMeshBc  # noqa: F401
+from .multidim import EulerCore, GradientElement, EulerField  # noqa: F401
 
-__all__ = ['onedim']
+__all__ = ['onedim', 'StaticMesh', 'StaticMeshBc', 'EulerCore',
+           'GradientElement', 'EulerField']

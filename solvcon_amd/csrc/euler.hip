@@ -1,0 +1,2215 @@
+/*
+ * Multidimensional Euler CESE march on the GPU: the device counterpart of
+ * solvcon's EulerCore (cpp/solvcon/multidim/euler.hpp:56-211 and
+ * euler_ce.cpp, euler_march.cpp, euler_solution.cpp, euler_gradient.cpp,
+ * euler_bc.cpp).
+ *
+ * Layout in HBM.  Body cells are grouped by cell type and, inside a group,
+ * ordered along a Morton curve of their centroids; ghost rows keep the
+ * reference order and come first.  A row is therefore 0..G-1 for ghosts and
+ * G + (internal position) for body cells.  Solution tables are
+ * structure-of-arrays, table[field * R + row] with R = G + N, so each
+ * field of a wavefront's cells is one contiguous load.  Each type group
+ * stores its CE geometry as geo[field * n + cell] (CE volumes, BCE
+ * centroids, sub-face centroids and normals, CFL distance) and its
+ * neighbour rows as nbr[face * n + cell].
+ *
+ * One substep of march() is one fused kernel per type group: the CE flux
+ * integral (calc_soln), the ghost update of every boundary face the cell
+ * owns (bc_soln from the pre-CFL state, bc_dsoln from so1c), the CFL and
+ * energy clamp (calc_cfl) and the gradient element with its weighting and
+ * limiter (calc_dsoln).  The neighbours' so0t is recomputed from the
+ * Jacobian that the flux integral builds anyway, so so0t is written only
+ * once at the end of march() (identical to the reference's last
+ * calc_solt).  The fine-grained stage methods run the same device code one
+ * stage at a time, reading so0t from memory the way the reference does.
+ *
+ * The API tables (cevol, cecnd, so0c, ...) keep the reference layout and
+ * numbering; they are produced and consumed by gather/scatter kernels.
+ */
+
+#include "euler_kernels.hpp"
+#include "mesh.hpp"
+#include "scan.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+namespace scamd
+{
+
+void sort_pairs_u64(uint64_t * kin, uint64_t * kout, int32_t * vin, int32_t * vout, int64_t n, hipStream_t s);
+
+namespace
+{
+
+constexpr unsigned BLK = 256;
+
+enum EulerArray : int
+{
+    EA_CEVOL = 0,
+    EA_CECND,
+    EA_SFCND,
+    EA_SFNML,
+    EA_SO0C,
+    EA_SO0N,
+    EA_SO0T,
+    EA_SO1C,
+    EA_SO1N,
+    EA_STM,
+    EA_CFLO,
+    EA_CFLC,
+    EA_GAMMA,
+    EA_COUNT,
+};
+
+enum Mode : int
+{
+    M_SOLN = 1,
+    M_BC = 2,
+    M_CFL = 4,
+    M_DSOLN = 8,
+    M_STMEM = 16,
+    M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
+};
+
+struct KP
+{
+    int64_t R;
+    int64_t G;
+    double hdt;
+    double qdt;
+    double sigma0;
+    double taumin;
+    double tauscale;
+    double const * s0c;
+    double const * s1c;
+    double * s0n;
+    double * s1n;
+    double * st;
+    double const * gam;
+    double * cflo;
+    double * cflc;
+    double const * cen;
+    int32_t const * gkind;
+    int32_t const * ginl;
+    double const * gmat;
+    double const * inlet;
+};
+
+struct GV
+{
+    int64_t off;
+    int64_t n;
+    int32_t const * nbr;
+    double const * geo;
+};
+
+template <typename T>
+__global__ void k_fill(T * a, int64_t n, T v)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        a[i] = v;
+    }
+}
+
+template <typename T>
+void fill(T * a, int64_t n, T v, hipStream_t s)
+{
+    if (n > 0)
+    {
+        k_fill<T><<<grid_for(n, BLK), BLK, 0, s>>>(a, n, v);
+        SC_HIP(hipGetLastError());
+    }
+}
+
+// --------------------------------------------------------------------------
+// The per-cell march kernel.
+// --------------------------------------------------------------------------
+
+template <int T, int MODE>
+__global__ void __launch_bounds__(BLK) k_cell(KP const p, GV const g)
+{
+    using TR = Traits<T>;
+    using LY = Layout<T>;
+    constexpr int ND = TR::ND;
+    constexpr int NF = TR::NF;
+    constexpr int NEQ = ND + 2;
+    constexpr int NFGE = TR::NFGE;
+
+    int64_t const i = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    if (i >= g.n)
+    {
+        return;
+    }
+    int64_t const n = g.n;
+    int64_t const R = p.R;
+    int64_t const row = p.G + g.off + i;
+    double const * geo = g.geo + i;
+
+    int64_t nb[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+    {
+        nb[f] = g.nbr[f * n + i];
+    }
+    // Neighbour solution points: CE centroid (body) or mesh centroid (ghost).
+    double jce[NF][ND];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+    {
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            jce[f][d] = p.cen[d * R + nb[f]];
+        }
+    }
+    double st[NF][NEQ];
+    double u0[NEQ];
+
+    if constexpr ((MODE & M_SOLN) != 0)
+    {
+        // calc_soln_impl (euler_march.cpp:219-302).
+        double acc[NEQ];
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+            acc[k] = 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+        {
+            int64_t const j = nb[f];
+            double js[NEQ];
+            double j1[NEQ][ND];
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+                js[k] = p.s0c[k * R + j];
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    j1[k][d] = p.s1c[(k * ND + d) * R + j];
+                }
+            }
+            double const bvol = geo[(LY::FV + 1 + f) * n];
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+                double fusp = js[k];
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    fusp += (geo[(LY::FB + f * ND + d) * n] - jce[f][d]) * j1[k][d];
+                }
+                acc[k] += fusp * bvol;
+            }
+            typename JacOf<ND>::type jac;
+            jac.update(p.gam[j], js);
+            if constexpr ((MODE & M_STMEM) != 0)
+            {
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    st[f][k] = p.st[k * R + j];
+                }
+            }
+            else
+            {
+                solt_of<ND>(jac, j1, st[f]);
+            }
+#pragma unroll
+            for (int s = 0; s < TR::nsub(f); ++s)
+            {
+                int const sf = TR::soff(f) + s;
+                double usfc[NEQ];
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    usfc[k] = p.qdt * st[f][k];
+#pragma unroll
+                    for (int d = 0; d < ND; ++d)
+                    {
+                        usfc[k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    double dot = 0.0;
+#pragma unroll
+                    for (int d2 = 0; d2 < ND; ++d2)
+                    {
+                        double dfcn = jac.f[k][d2];
+#pragma unroll
+                        for (int m = 0; m < NEQ; ++m)
+                        {
+                            dfcn += jac.j[k][m][d2] * usfc[m];
+                        }
+                        dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
+                    }
+                    acc[k] -= p.hdt * dot;
+                }
+            }
+        }
+        double const cvol = geo[LY::FV * n];
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+            u0[k] = acc[k] / cvol;
+        }
+    }
+
+    if constexpr ((MODE & M_BC) != 0)
+    {
+        // bc_soln (from the pre-CFL state) and bc_dsoln (from so1c) of the
+        // boundary faces this cell owns (euler_bc.cpp:143-320).
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+        {
+            int64_t const j = nb[f];
+            if (j < p.G)
+            {
+                int32_t const kind = p.gkind[j];
+                if (kind >= 0)
+                {
+                    double mat[ND][ND];
+#pragma unroll
+                    for (int a = 0; a < ND; ++a)
+                    {
+#pragma unroll
+                        for (int b = 0; b < ND; ++b)
+                        {
+                            mat[a][b] = p.gmat[(a * ND + b) * p.G + j];
+                        }
+                    }
+                    double o0[NEQ];
+                    bc_order0<ND>(kind, mat, p.inlet + static_cast<int64_t>(p.ginl[j]) * (ND + 3), u0, o0);
+#pragma unroll
+                    for (int k = 0; k < NEQ; ++k)
+                    {
+                        p.s0n[k * R + j] = o0[k];
+                    }
+                    double in1[NEQ][ND];
+#pragma unroll
+                    for (int k = 0; k < NEQ; ++k)
+                    {
+#pragma unroll
+                        for (int d = 0; d < ND; ++d)
+                        {
+                            in1[k][d] = p.s1c[(k * ND + d) * R + row];
+                        }
+                    }
+                    double o1[NEQ][ND];
+                    bc_order1<ND>(kind, mat, in1, o1);
+#pragma unroll
+                    for (int k = 0; k < NEQ; ++k)
+                    {
+#pragma unroll
+                        for (int d = 0; d < ND; ++d)
+                        {
+                            p.s1n[(k * ND + d) * R + j] = o1[k][d];
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    double cflc_v = 0.0;
+    if constexpr ((MODE & M_CFL) != 0)
+    {
+        // calc_cfl (euler_solution.cpp:114-159).
+        if constexpr ((MODE & M_SOLN) == 0)
+        {
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+                u0[k] = p.s0n[k * R + row];
+            }
+        }
+        double const dist = geo[LY::FD * n];
+        double const ga = p.gam[row];
+        double const ga1 = ga - 1.0;
+        double const density = u0[0];
+        double momsq = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            double const mom = u0[1 + d];
+            momsq += mom * mom;
+        }
+        double const ke = momsq / (2.0 * density);
+        double const energy = u0[NEQ - 1];
+        double const pr = ga1 * (energy - ke);
+        double const pra = (pr + fabs(pr)) / 2.0;
+        double const wspd = sqrt(ga * pra / density) + sqrt(momsq) / density;
+        double const co = p.hdt * wspd / dist;
+        p.cflo[row] = co;
+        cflc_v = (co - 1.0) * pra / (pra + TINYJ) + 1.0;
+        p.cflc[row] = cflc_v;
+        u0[NEQ - 1] = pra / ga1 + ke + TINYJ;
+    }
+    if constexpr ((MODE & (M_SOLN | M_CFL)) != 0)
+    {
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+            p.s0n[k * R + row] = u0[k];
+        }
+    }
+
+    if constexpr ((MODE & M_DSOLN) != 0)
+    {
+        // calc_dsoln (euler_gradient.cpp:23-130) with GradientElement
+        // (GradientElement.cpp:137-213).
+        if constexpr ((MODE & M_CFL) == 0)
+        {
+            cflc_v = p.cflc[row];
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+                u0[k] = p.s0n[k * R + row];
+            }
+        }
+        if constexpr ((MODE & M_SOLN) == 0)
+        {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+            {
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    st[f][k] = p.st[k * R + nb[f]];
+                }
+            }
+        }
+        double const acfl = fabs(cflc_v);
+        double const sgm0 = p.sigma0 / acfl;
+        double const tau = p.taumin + acfl * p.tauscale;
+        double ice[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            ice[d] = p.cen[d * R + row];
+        }
+        double gp[NF][ND];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+        {
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                double const mid = geo[(LY::FB + f * ND + d) * n];
+                gp[f][d] = mid + tau * (jce[f][d] - mid);
+            }
+        }
+        double avg[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            avg[d] = 0.0;
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+        {
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                avg[d] += gp[f][d];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            avg[d] /= NF;
+        }
+        // calc_gge_centroid (GradientElement.cpp:81-133).
+        double cnd[ND];
+        double voc = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            cnd[d] = 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < NFGE; ++e)
+        {
+            double sub[ND];
+            double dst[ND][ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                sub[d] = avg[d];
+            }
+#pragma unroll
+            for (int v = 0; v < ND; ++v)
+            {
+                int const fl = TR::fge(e, v) - 1;
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    sub[d] += gp[fl][d];
+                    dst[v][d] = gp[fl][d] - avg[d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                sub[d] /= static_cast<double>(ND + 1);
+            }
+            double const vob = ge_det<ND>(dst);
+            voc += vob;
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                cnd[d] += sub[d] * vob;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            cnd[d] /= voc;
+        }
+        double idis[NF][ND];
+        double udf[NF][NEQ];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+        {
+            double jdis[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                idis[f][d] = gp[f][d] - cnd[d];
+                jdis[d] = (gp[f][d] - jce[f][d]) + ice[d] - cnd[d];
+            }
+            int64_t const j = nb[f];
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+                double val = p.s0c[k * R + j] + p.hdt * st[f][k] - u0[k];
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    val += jdis[d] * p.s1c[(k * ND + d) * R + j];
+                }
+                udf[f][k] = val;
+            }
+        }
+        // Displacement-matrix adjugates and determinants of every FGE;
+        // they depend only on geometry, so each is formed once.
+        double adj[NFGE][ND][ND];
+        double det[NFGE];
+#pragma unroll
+        for (int e = 0; e < NFGE; ++e)
+        {
+            double dst[ND][ND];
+#pragma unroll
+            for (int v = 0; v < ND; ++v)
+            {
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    dst[v][d] = idis[TR::fge(e, v) - 1][d];
+                }
+            }
+            ge_adj<ND>(dst, adj[e]);
+            det[e] = ge_det<ND>(dst);
+        }
+        constexpr double ofg1 = 1.0 / NFGE;
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+            double gk[NFGE][ND];
+            double wk[NFGE];
+            double wacc = 0.0;
+#pragma unroll
+            for (int e = 0; e < NFGE; ++e)
+            {
+                double r[ND];
+#pragma unroll
+                for (int a = 0; a < ND; ++a)
+                {
+                    r[a] = 0.0;
+#pragma unroll
+                    for (int b = 0; b < ND; ++b)
+                    {
+                        r[a] += adj[e][a][b] * udf[TR::fge(e, b) - 1][k];
+                    }
+                }
+                double sq = 0.0;
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    gk[e][d] = r[d] / det[e];
+                    sq += gk[e][d] * gk[e][d];
+                }
+                double const wgt = 1.0 / sqrt(sq + ALMOST_ZERO);
+                wacc += wgt;
+                wk[e] = wgt;
+            }
+            double wmax = 0.0;
+            double wmin = 0.0;
+#pragma unroll
+            for (int e = 0; e < NFGE; ++e)
+            {
+                double const w = wk[e] / wacc - ofg1;
+                wk[e] = w;
+                wmax = fmax(wmax, w);
+                wmin = fmin(wmin, w);
+            }
+            double const sm = fmin((1.0 - ofg1) / (wmax + ALMOST_ZERO), -ofg1 / (wmin - ALMOST_ZERO));
+            double const smax = fmin(sm, sgm0);
+            double out[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                out[d] = 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < NFGE; ++e)
+            {
+                double const w = ofg1 + smax * wk[e];
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    out[d] += w * gk[e][d];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                p.s1n[(k * ND + d) * R + row] = out[d];
+            }
+        }
+    }
+}
+
+// calc_solt over every row, ghosts included (euler_march.cpp:161-191).
+template <int ND>
+__global__ void k_solt(KP const p)
+{
+    constexpr int NEQ = ND + 2;
+    int64_t const r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= p.R)
+    {
+        return;
+    }
+    int64_t const R = p.R;
+    double s0[NEQ];
+    double s1[NEQ][ND];
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k)
+    {
+        s0[k] = p.s0c[k * R + r];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            s1[k][d] = p.s1c[(k * ND + d) * R + r];
+        }
+    }
+    typename JacOf<ND>::type jac;
+    jac.update(p.gam[r], s0);
+    double st[NEQ];
+    solt_of<ND>(jac, s1, st);
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k)
+    {
+        p.st[k * R + r] = st[k];
+    }
+}
+
+// Ghost updates for the stand-alone bc_soln / bc_dsoln (euler_bc.cpp:143-320).
+template <int ND, bool ORDER1>
+__global__ void k_bc(KP const p, int32_t const * gsrc)
+{
+    constexpr int NEQ = ND + 2;
+    int64_t const j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= p.G)
+    {
+        return;
+    }
+    int32_t const kind = p.gkind[j];
+    if (kind < 0)
+    {
+        return;
+    }
+    int64_t const R = p.R;
+    int64_t const src = gsrc[j];
+    double mat[ND][ND];
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+    {
+#pragma unroll
+        for (int b = 0; b < ND; ++b)
+        {
+            mat[a][b] = p.gmat[(a * ND + b) * p.G + j];
+        }
+    }
+    if constexpr (!ORDER1)
+    {
+        double in[NEQ];
+        double out[NEQ];
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+            in[k] = p.s0n[k * R + src];
+        }
+        bc_order0<ND>(kind, mat, p.inlet + static_cast<int64_t>(p.ginl[j]) * (ND + 3), in, out);
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+            p.s0n[k * R + j] = out[k];
+        }
+    }
+    else
+    {
+        double in[NEQ][ND];
+        double out[NEQ][ND];
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                in[k][d] = p.s1c[(k * ND + d) * R + src];
+            }
+        }
+        bc_order1<ND>(kind, mat, in, out);
+#pragma unroll
+        for (int k = 0; k < NEQ; ++k)
+        {
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                p.s1n[(k * ND + d) * R + j] = out[k][d];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// CE geometry (prepare_ce_2d / prepare_ce_3d, euler_ce.cpp:78-347).
+// --------------------------------------------------------------------------
+
+struct MeshView
+{
+    int32_t const * clfcs;
+    int32_t const * fccls;
+    int32_t const * fcnds;
+    double const * ndcrd;
+    double const * clcnd;
+    double const * fccnd;
+    int64_t G;
+    int32_t const * int_of_ref;
+    int32_t const * ref_of_int;
+};
+
+template <int T>
+__global__ void k_prepare(MeshView const m, int64_t off, int64_t n, int32_t * nbr, double * geo, double * cen, int64_t R)
+{
+    using TR = Traits<T>;
+    using LY = Layout<T>;
+    constexpr int ND = TR::ND;
+    constexpr int NF = TR::NF;
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+    {
+        return;
+    }
+    int32_t const icl = m.ref_of_int[off + i];
+    int32_t const * cf = m.clfcs + static_cast<int64_t>(icl) * (CLMFC + 1);
+    double * gi = geo + i;
+    double volc = 0.0;
+    double cc[ND];
+    double crdi[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+    {
+        cc[d] = 0.0;
+        crdi[d] = m.clcnd[static_cast<int64_t>(icl) * ND + d];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+    {
+        int32_t const ifc = cf[f + 1];
+        int32_t const * fc = m.fccls + static_cast<int64_t>(ifc) * FCREL;
+        int32_t const jcl = fc[0] + fc[1] - icl;
+        nbr[f * n + i] = jcl < 0 ? static_cast<int32_t>(m.G + jcl) : static_cast<int32_t>(m.G + m.int_of_ref[jcl]);
+        double crde[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            crde[d] = m.clcnd[static_cast<int64_t>(jcl) * ND + d];
+        }
+        int32_t const * fn = m.fcnds + static_cast<int64_t>(ifc) * (FCMND + 1);
+        if constexpr (ND == 2)
+        {
+            double const crd10 = m.ndcrd[static_cast<int64_t>(fn[1]) * 2];
+            double const crd11 = m.ndcrd[static_cast<int64_t>(fn[1]) * 2 + 1];
+            double const crd20 = m.ndcrd[static_cast<int64_t>(fn[2]) * 2];
+            double const crd21 = m.ndcrd[static_cast<int64_t>(fn[2]) * 2 + 1];
+            double const cndi0 = (crd10 + crd20 + crdi[0]) / 3.0;
+            double const cndi1 = (crd11 + crd21 + crdi[1]) / 3.0;
+            double const voli = fabs((crd10 - crdi[0]) * (crd21 - crdi[1]) - (crd11 - crdi[1]) * (crd20 - crdi[0])) / 2.0;
+            double const cnde0 = (crd10 + crd20 + crde[0]) / 3.0;
+            double const cnde1 = (crd11 + crd21 + crde[1]) / 3.0;
+            double const vole = fabs((crd10 - crde[0]) * (crd21 - crde[1]) - (crd11 - crde[1]) * (crd20 - crde[0])) / 2.0;
+            double const volb = voli + vole;
+            gi[(LY::FV + 1 + f) * n] = volb;
+            gi[(LY::FB + f * 2) * n] = (cndi0 * voli + cnde0 * vole) / volb;
+            gi[(LY::FB + f * 2 + 1) * n] = (cndi1 * voli + cnde1 * vole) / volb;
+            volc += volb;
+            cc[0] += (cndi0 * voli + cnde0 * vole);
+            cc[1] += (cndi1 * voli + cnde1 * vole);
+            double const cross = (crd10 - crde[0]) * (crd21 - crde[1]) - (crd11 - crde[1]) * (crd20 - crde[0]);
+            double const sgn = (cross > 0.0) ? 1.0 : -1.0;
+            int const s0 = TR::soff(f);
+            gi[(LY::FC + s0 * 2) * n] = (crde[0] + crd10) / 2.0;
+            gi[(LY::FC + s0 * 2 + 1) * n] = (crde[1] + crd11) / 2.0;
+            double const s0dx = crde[0] - crd10;
+            double const s0dy = crde[1] - crd11;
+            gi[(LY::FN + s0 * 2) * n] = -s0dy * sgn;
+            gi[(LY::FN + s0 * 2 + 1) * n] = s0dx * sgn;
+            gi[(LY::FC + (s0 + 1) * 2) * n] = (crd20 + crde[0]) / 2.0;
+            gi[(LY::FC + (s0 + 1) * 2 + 1) * n] = (crd21 + crde[1]) / 2.0;
+            double const s1dx = crd20 - crde[0];
+            double const s1dy = crd21 - crde[1];
+            gi[(LY::FN + (s0 + 1) * 2) * n] = -s1dy * sgn;
+            gi[(LY::FN + (s0 + 1) * 2 + 1) * n] = s1dx * sgn;
+        }
+        else
+        {
+            constexpr int MAXN = 4;
+            int const nnd = TR::nsub(f);
+            double pfc[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                pfc[d] = m.fccnd[static_cast<int64_t>(ifc) * 3 + d];
+            }
+            double crd[MAXN + 1][3];
+#pragma unroll
+            for (int k = 0; k < MAXN; ++k)
+            {
+                if (k < nnd)
+                {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        crd[k][d] = m.ndcrd[static_cast<int64_t>(fn[k + 1]) * 3 + d];
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                crd[nnd][d] = crd[0][d];
+            }
+            double volb = 0.0;
+            double bc[3] = {0.0, 0.0, 0.0};
+            double const voe = (fc[0] == icl) ? 0.5 : -0.5;
+#pragma unroll
+            for (int k = 0; k < MAXN; ++k)
+            {
+                if (k >= nnd)
+                {
+                    break;
+                }
+                double const u0 = crd[k][0] - pfc[0];
+                double const u1 = crd[k][1] - pfc[1];
+                double const u2 = crd[k][2] - pfc[2];
+                double const v0 = crd[k + 1][0] - pfc[0];
+                double const v1 = crd[k + 1][1] - pfc[1];
+                double const v2 = crd[k + 1][2] - pfc[2];
+                double const w0 = u1 * v2 - u2 * v1;
+                double const w1 = u2 * v0 - u0 * v2;
+                double const w2 = u0 * v1 - u1 * v0;
+                double const voli = fabs(w0 * (crdi[0] - pfc[0]) + w1 * (crdi[1] - pfc[1]) + w2 * (crdi[2] - pfc[2])) / 6.0;
+                double const vole = fabs(w0 * (crde[0] - pfc[0]) + w1 * (crde[1] - pfc[1]) + w2 * (crde[2] - pfc[2])) / 6.0;
+                volb += voli + vole;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    double const cndi = (crd[k][d] + crd[k + 1][d] + pfc[d] + crdi[d]) / 4.0;
+                    double const cnde = (crd[k][d] + crd[k + 1][d] + pfc[d] + crde[d]) / 4.0;
+                    bc[d] += cndi * voli + cnde * vole;
+                }
+                int const si = TR::soff(f) + k;
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                {
+                    gi[(LY::FC + si * 3 + d) * n] = (crde[d] + crd[k][d] + crd[k + 1][d]) / 3.0;
+                }
+                double const su0 = crd[k][0] - crde[0];
+                double const su1 = crd[k][1] - crde[1];
+                double const su2 = crd[k][2] - crde[2];
+                double const sv0 = crd[k + 1][0] - crde[0];
+                double const sv1 = crd[k + 1][1] - crde[1];
+                double const sv2 = crd[k + 1][2] - crde[2];
+                gi[(LY::FN + si * 3) * n] = (su1 * sv2 - su2 * sv1) * voe;
+                gi[(LY::FN + si * 3 + 1) * n] = (su2 * sv0 - su0 * sv2) * voe;
+                gi[(LY::FN + si * 3 + 2) * n] = (su0 * sv1 - su1 * sv0) * voe;
+            }
+            gi[(LY::FV + 1 + f) * n] = volb;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                gi[(LY::FB + f * 3 + d) * n] = bc[d] / volb;
+            }
+            volc += volb;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                cc[d] += bc[d];
+            }
+        }
+    }
+    gi[LY::FV * n] = volc;
+    int64_t const row = m.G + off + i;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+    {
+        cen[d * R + row] = cc[d] / volc;
+    }
+}
+
+// Minimum CCE-to-BCE centroid distance of calc_cfl (euler_solution.cpp:118-130);
+// it depends on geometry only, so it is evaluated once per geometry change.
+template <int T>
+__global__ void k_dist(int64_t G, int64_t off, int64_t n, double * geo, double const * cen, int64_t R)
+{
+    using TR = Traits<T>;
+    using LY = Layout<T>;
+    constexpr int ND = TR::ND;
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+    {
+        return;
+    }
+    int64_t const row = G + off + i;
+    double dist = DBL_MAX;
+#pragma unroll
+    for (int f = 0; f < TR::NF; ++f)
+    {
+        double d2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+        {
+            double const diff = geo[(LY::FB + f * ND + d) * n + i] - cen[d * R + row];
+            d2 += diff * diff;
+        }
+        double const s = sqrt(d2);
+        dist = (s < dist) ? s : dist;
+    }
+    geo[LY::FD * n + i] = dist;
+}
+
+// Ghost rows of the solution-point table: the mesh centroid of each ghost.
+__global__ void k_cen_ghost(int ndim, int64_t G, double const * clcnd_full, double * cen, int64_t R)
+{
+    int64_t const r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < G)
+    {
+        for (int d = 0; d < ndim; ++d)
+        {
+            cen[d * R + r] = clcnd_full[r * ndim + d];
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Layout conversion between the API tables (reference order, row-major)
+// and the internal tables.
+// --------------------------------------------------------------------------
+
+__device__ inline int64_t irow_of(int64_t r, int64_t G, int32_t const * int_of_ref)
+{
+    return r < G ? r : G + int_of_ref[r - G];
+}
+
+// Row-major [R][w] <-> structure-of-arrays [w][R].
+template <bool DOWN>
+__global__ void k_state(int64_t R, int64_t G, int w, int32_t const * int_of_ref, double * api, double * soa)
+{
+    int64_t const r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R)
+    {
+        return;
+    }
+    int64_t const ir = irow_of(r, G, int_of_ref);
+    for (int c = 0; c < w; ++c)
+    {
+        if (DOWN)
+        {
+            api[r * w + c] = soa[c * R + ir];
+        }
+        else
+        {
+            soa[c * R + ir] = api[r * w + c];
+        }
+    }
+}
+
+// CE geometry of one type group <-> cevol/cecnd/sfcnd/sfnml.
+template <int T, bool DOWN>
+__global__ void k_geom(int which, int64_t G, int64_t off, int64_t n, int32_t const * ref_of_int, double * api,
+                       double * geo, double * cen, int64_t R)
+{
+    using TR = Traits<T>;
+    using LY = Layout<T>;
+    constexpr int ND = TR::ND;
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+    {
+        return;
+    }
+    int64_t const r = ref_of_int[off + i];
+    int64_t const row = G + off + i;
+    if (which == EA_CEVOL)
+    {
+        double * a = api + (G + r) * (CLMFC + 1);
+        for (int c = 0; c <= TR::NF; ++c)
+        {
+            if (DOWN)
+            {
+                a[c] = geo[(LY::FV + c) * n + i];
+            }
+            else
+            {
+                geo[(LY::FV + c) * n + i] = a[c];
+            }
+        }
+    }
+    else if (which == EA_CECND)
+    {
+        double * a = api + (G + r) * (CLMFC + 1) * ND;
+        for (int d = 0; d < ND; ++d)
+        {
+            if (DOWN)
+            {
+                a[d] = cen[d * R + row];
+            }
+            else
+            {
+                cen[d * R + row] = a[d];
+            }
+        }
+        for (int c = 0; c < TR::NF * ND; ++c)
+        {
+            if (DOWN)
+            {
+                a[ND + c] = geo[(LY::FB + c) * n + i];
+            }
+            else
+            {
+                geo[(LY::FB + c) * n + i] = a[ND + c];
+            }
+        }
+    }
+    else
+    {
+        int const base = which == EA_SFCND ? LY::FC : LY::FN;
+        double * a = api + r * (CLMFC * FCMND) * ND;
+        for (int f = 0; f < TR::NF; ++f)
+        {
+            for (int s = 0; s < TR::nsub(f); ++s)
+            {
+                for (int d = 0; d < ND; ++d)
+                {
+                    double & x = a[(f * FCMND + s) * ND + d];
+                    double & y = geo[(base + (TR::soff(f) + s) * ND + d) * n + i];
+                    if (DOWN)
+                    {
+                        x = y;
+                    }
+                    else
+                    {
+                        y = x;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// init_solution (euler_solution.cpp:79-98): gamma on every row, the uniform
+// conserved state on the body rows of so0n.
+__global__ void k_init(int64_t R, int64_t G, int neq, int ndim, double ga, double rho, double v0, double v1, double v2,
+                       double energy, double * gam, double * s0n)
+{
+    int64_t const r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R)
+    {
+        return;
+    }
+    gam[r] = ga;
+    if (r >= G)
+    {
+        double const v[3] = {v0, v1, v2};
+        s0n[r] = rho;
+        for (int d = 0; d < ndim; ++d)
+        {
+            s0n[(1 + d) * R + r] = rho * v[d];
+        }
+        s0n[(neq - 1) * R + r] = energy;
+    }
+}
+
+// Morton code of the centroid for the cell ordering; the cell type sits in
+// the top four bits so that the sort also groups the types.
+__device__ inline uint64_t spread2(uint64_t x)
+{
+    x &= 0x3fffffffULL;
+    x = (x | (x << 16)) & 0x0000ffff0000ffffULL;
+    x = (x | (x << 8)) & 0x00ff00ff00ff00ffULL;
+    x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0fULL;
+    x = (x | (x << 2)) & 0x3333333333333333ULL;
+    x = (x | (x << 1)) & 0x5555555555555555ULL;
+    return x;
+}
+
+__device__ inline uint64_t spread3(uint64_t x)
+{
+    x &= 0xfffffULL;
+    x = (x | (x << 32)) & 0x00000000ffff0000ULL ? (x | (x << 32)) & 0x001f00000000ffffULL : (x | (x << 32)) & 0x001f00000000ffffULL;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffULL;
+    x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+    x = (x | (x << 2)) & 0x1249249249249249ULL;
+    return x;
+}
+
+__global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double const * clcnd, double const * lo,
+                       double const * hi, int use_sfc, uint64_t * keys, int32_t * vals)
+{
+    int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncell)
+    {
+        return;
+    }
+    uint64_t code = 0;
+    if (use_sfc)
+    {
+        int const bits = ndim == 2 ? 30 : 20;
+        double const scale = static_cast<double>((1ULL << bits) - 1);
+        uint64_t q[3] = {0, 0, 0};
+        for (int d = 0; d < ndim; ++d)
+        {
+            double const w = hi[d] - lo[d];
+            double t = w > 0 ? (clcnd[static_cast<int64_t>(i) * ndim + d] - lo[d]) / w : 0.0;
+            t = t < 0 ? 0 : (t > 1 ? 1 : t);
+            q[d] = static_cast<uint64_t>(t * scale);
+        }
+        code = ndim == 2 ? (spread2(q[0]) | (spread2(q[1]) << 1))
+                         : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+    }
+    keys[i] = (static_cast<uint64_t>(cltpn[i] & 0xf) << 60) | (code & 0x0fffffffffffffffULL);
+    vals[i] = i;
+}
+
+__global__ void k_bbox(int ndim, int32_t ncell, double const * clcnd, double * out)
+{
+    // One workgroup reduces everything: the bounding box is set-up work.
+    __shared__ double lo[BLK][3];
+    __shared__ double hi[BLK][3];
+    for (int d = 0; d < 3; ++d)
+    {
+        lo[threadIdx.x][d] = DBL_MAX;
+        hi[threadIdx.x][d] = -DBL_MAX;
+    }
+    for (int64_t i = threadIdx.x; i < ncell; i += blockDim.x)
+    {
+        for (int d = 0; d < ndim; ++d)
+        {
+            double const x = clcnd[i * ndim + d];
+            lo[threadIdx.x][d] = fmin(lo[threadIdx.x][d], x);
+            hi[threadIdx.x][d] = fmax(hi[threadIdx.x][d], x);
+        }
+    }
+    __syncthreads();
+    for (unsigned s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+        {
+            for (int d = 0; d < 3; ++d)
+            {
+                lo[threadIdx.x][d] = fmin(lo[threadIdx.x][d], lo[threadIdx.x + s][d]);
+                hi[threadIdx.x][d] = fmax(hi[threadIdx.x][d], hi[threadIdx.x + s][d]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            out[d] = lo[0][d];
+            out[3 + d] = hi[0][d];
+        }
+    }
+}
+
+__global__ void k_type_count(int32_t ncell, int32_t const * cltpn, int32_t * cnt)
+{
+    int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncell)
+    {
+        int32_t const t = cltpn[i];
+        atomicAdd(&cnt[(t >= 0 && t <= 8) ? t : 0], 1);
+    }
+}
+
+__global__ void k_invert(int32_t n, int32_t const * ref_of_int, int32_t * int_of_ref)
+{
+    int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        int_of_ref[ref_of_int[i]] = i;
+    }
+}
+
+// Orthonormal frame with the outward unit normal as its first row
+// (normal_matrix<2>/<3>, euler_bc.cpp:87-128).
+__device__ void normal_frame(int ndim, double const * n, double (&m)[3][3])
+{
+    if (ndim == 2)
+    {
+        m[0][0] = n[0];
+        m[0][1] = n[1];
+        m[1][0] = -n[1];
+        m[1][1] = n[0];
+        return;
+    }
+    double ax[3] = {0.0, 0.0, 0.0};
+    double const a0 = fabs(n[0]);
+    double const a1 = fabs(n[1]);
+    double const a2 = fabs(n[2]);
+    if (a0 <= a1 && a0 <= a2)
+    {
+        ax[0] = 1.0;
+    }
+    else if (a1 <= a2)
+    {
+        ax[1] = 1.0;
+    }
+    else
+    {
+        ax[2] = 1.0;
+    }
+    double t1[3] = {n[1] * ax[2] - n[2] * ax[1], n[2] * ax[0] - n[0] * ax[2], n[0] * ax[1] - n[1] * ax[0]};
+    double const len = sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+    t1[0] = t1[0] / len;
+    t1[1] = t1[1] / len;
+    t1[2] = t1[2] / len;
+    double const t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+    for (int d = 0; d < 3; ++d)
+    {
+        m[0][d] = n[d];
+        m[1][d] = t1[d];
+        m[2][d] = t2[d];
+    }
+}
+
+// Boundary tables per ghost row: the interior row and the face frame.
+__global__ void k_ghost_meta(int ndim, int64_t G, int32_t const * bndfcs, int32_t const * fccls_b,
+                             double const * fcnml_b, int32_t const * int_of_ref, int32_t * gsrc, double * gmat)
+{
+    int64_t const j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= G)
+    {
+        return;
+    }
+    // Ghost row j is ghost cell -(G - j), which mirrors bndfcs row G-j-1.
+    int64_t const k = G - 1 - j;
+    int32_t const ifc = bndfcs[k * BFREL];
+    int32_t const icl = fccls_b[static_cast<int64_t>(ifc) * FCREL];
+    gsrc[j] = static_cast<int32_t>(G + int_of_ref[icl]);
+    double n[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < ndim; ++d)
+    {
+        n[d] = fcnml_b[static_cast<int64_t>(ifc) * ndim + d];
+    }
+    double m[3][3];
+    normal_frame(ndim, n, m);
+    for (int a = 0; a < ndim; ++a)
+    {
+        for (int b = 0; b < ndim; ++b)
+        {
+            gmat[(a * ndim + b) * G + j] = m[a][b];
+        }
+    }
+}
+
+__global__ void k_face_frame(int ndim, int32_t ifc, double const * fcnml_b, double * out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+    {
+        return;
+    }
+    double n[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < ndim; ++d)
+    {
+        n[d] = fcnml_b[static_cast<int64_t>(ifc) * ndim + d];
+    }
+    double m[3][3];
+    normal_frame(ndim, n, m);
+    for (int a = 0; a < ndim; ++a)
+    {
+        for (int b = 0; b < ndim; ++b)
+        {
+            out[a * ndim + b] = m[a][b];
+        }
+    }
+}
+
+__global__ void k_set_bc(int32_t n, int32_t const * faces, int32_t const * fccls_b, int64_t G, int32_t kind,
+                         int32_t inl, int32_t * gkind, int32_t * ginl)
+{
+    int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        int32_t const gcl = fccls_b[static_cast<int64_t>(faces[i]) * FCREL + 1];
+        int64_t const j = G + gcl;
+        gkind[j] = kind;
+        ginl[j] = inl;
+    }
+}
+
+// Sums over the body cells for diagnostics: density * volume and the CFL
+// range (EulerField.calc_overall_mass / calc_cfl_range).
+__global__ void k_reduce(int64_t R, int64_t G, double const * s0n, double const * clvol_b, int32_t const * ref_of_int,
+                         double const * cflo, double * part)
+{
+    __shared__ double sm[BLK];
+    __shared__ double mn[BLK];
+    __shared__ double mx[BLK];
+    double a = 0.0;
+    double lo = DBL_MAX;
+    double hi = -DBL_MAX;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < R - G;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    {
+        a += s0n[G + i] * clvol_b[ref_of_int[i]];
+        lo = fmin(lo, cflo[G + i]);
+        hi = fmax(hi, cflo[G + i]);
+    }
+    sm[threadIdx.x] = a;
+    mn[threadIdx.x] = lo;
+    mx[threadIdx.x] = hi;
+    __syncthreads();
+    for (unsigned s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+        {
+            sm[threadIdx.x] += sm[threadIdx.x + s];
+            mn[threadIdx.x] = fmin(mn[threadIdx.x], mn[threadIdx.x + s]);
+            mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        part[blockIdx.x * 3] = sm[0];
+        part[blockIdx.x * 3 + 1] = mn[0];
+        part[blockIdx.x * 3 + 2] = mx[0];
+    }
+}
+
+} /* end namespace */
+
+struct EulerBoundary
+{
+    int32_t kind;
+    std::vector<int32_t> faces;
+    std::vector<double> value;
+};
+
+struct DEuler
+{
+    DMesh * mesh = nullptr;
+    int ndim = 0;
+    int neq = 0;
+    int64_t G = 0;
+    int64_t N = 0;
+    int64_t R = 0;
+    double dt = 0.0;
+    double sigma0 = 3.0; // euler.hpp:183-188
+    double taumin = 0.0;
+    double tauscale = 1.0;
+    bool use_sfc = true;
+    hipStream_t stream = nullptr;
+
+    struct Group
+    {
+        int32_t type = 0;
+        int64_t off = 0;
+        int64_t n = 0;
+        DevBuf<int32_t> nbr;
+        DevBuf<double> geo;
+    };
+    std::vector<Group> groups;
+    DevBuf<int32_t> int_of_ref, ref_of_int;
+    DevBuf<double> cen, s0[2], s1[2], st, stm, gam, cflo, cflc;
+    int cur = 0;
+
+    std::vector<EulerBoundary> bcs;
+    DevBuf<int32_t> gkind, ginl, gsrc;
+    DevBuf<double> gmat, inlet;
+
+    DEuler(DMesh * m, double time_increment, bool sfc);
+    ~DEuler()
+    {
+        if (stream != nullptr)
+        {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    void sync() { SC_HIP(hipStreamSynchronize(stream)); }
+
+    KP params(bool swap_after = false) const
+    {
+        (void)swap_after;
+        KP p;
+        p.R = R;
+        p.G = G;
+        p.hdt = dt * 0.5;
+        p.qdt = dt * 0.25;
+        p.sigma0 = sigma0;
+        p.taumin = taumin;
+        p.tauscale = tauscale;
+        p.s0c = s0[cur].ptr;
+        p.s1c = s1[cur].ptr;
+        p.s0n = s0[1 - cur].ptr;
+        p.s1n = s1[1 - cur].ptr;
+        p.st = st.ptr;
+        p.gam = gam.ptr;
+        p.cflo = cflo.ptr;
+        p.cflc = cflc.ptr;
+        p.cen = cen.ptr;
+        p.gkind = gkind.ptr;
+        p.ginl = ginl.ptr;
+        p.gmat = gmat.ptr;
+        p.inlet = inlet.ptr;
+        return p;
+    }
+
+    template <int MODE>
+    void run_cells()
+    {
+        KP const p = params();
+        for (auto & gr : groups)
+        {
+            if (gr.n == 0)
+            {
+                continue;
+            }
+            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr};
+            unsigned const nb = grid_for(gr.n, BLK);
+            switch (gr.type)
+            {
+            case CT_TRI: k_cell<CT_TRI, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_QUAD: k_cell<CT_QUAD, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TET: k_cell<CT_TET, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_HEX: k_cell<CT_HEX, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PRISM: k_cell<CT_PRISM, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PYRAMID: k_cell<CT_PYRAMID, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+            default: fail(SC_ERUNTIME, "EulerCore: unsupported cell type in group");
+            }
+            SC_HIP(hipGetLastError());
+        }
+    }
+
+    void calc_solt()
+    {
+        KP const p = params();
+        if (R == 0)
+        {
+            return;
+        }
+        if (ndim == 2)
+        {
+            k_solt<2><<<grid_for(R, BLK), BLK, 0, stream>>>(p);
+        }
+        else
+        {
+            k_solt<3><<<grid_for(R, BLK), BLK, 0, stream>>>(p);
+        }
+        SC_HIP(hipGetLastError());
+    }
+
+    void bc_soln()
+    {
+        if (G == 0)
+        {
+            return;
+        }
+        KP const p = params();
+        if (ndim == 2)
+        {
+            k_bc<2, false><<<grid_for(G, BLK), BLK, 0, stream>>>(p, gsrc.ptr);
+        }
+        else
+        {
+            k_bc<3, false><<<grid_for(G, BLK), BLK, 0, stream>>>(p, gsrc.ptr);
+        }
+        SC_HIP(hipGetLastError());
+    }
+
+    void bc_dsoln()
+    {
+        if (G == 0)
+        {
+            return;
+        }
+        KP const p = params();
+        if (ndim == 2)
+        {
+            k_bc<2, true><<<grid_for(G, BLK), BLK, 0, stream>>>(p, gsrc.ptr);
+        }
+        else
+        {
+            k_bc<3, true><<<grid_for(G, BLK), BLK, 0, stream>>>(p, gsrc.ptr);
+        }
+        SC_HIP(hipGetLastError());
+    }
+
+    void update() { cur = 1 - cur; }
+
+    void substep_fused()
+    {
+        update();
+        run_cells<M_FUSED>();
+    }
+
+    void march(int64_t steps)
+    {
+        for (int64_t it = 0; it < steps; ++it)
+        {
+            substep_fused();
+            substep_fused();
+        }
+        if (steps > 0)
+        {
+            calc_solt();
+        }
+    }
+
+    void march_substep()
+    {
+        substep_fused();
+        calc_solt();
+    }
+
+    void prepare_ce();
+    void init_solution(double ga, double rho, double const * v, double p);
+    void add_bc(int32_t kind, std::vector<int32_t> const & faces, std::vector<double> const & value);
+    void refresh_bc();
+    void table(int id, int64_t & rows, int64_t & w, int64_t & ngh, int & rank);
+    void copy(int id, double * host, int64_t count, bool down);
+};
+
+DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
+    : mesh(m)
+    , dt(time_increment)
+    , use_sfc(sfc)
+{
+    if (m->ndim != 2 && m->ndim != 3)
+    {
+        fail(SC_EVALUE, "EulerCore: ndim must be 2 or 3");
+    }
+    SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    ndim = m->ndim;
+    neq = ndim + 2;
+    G = m->ngstcell;
+    N = m->ncell;
+    R = G + N;
+    hipStream_t const s = stream;
+    m->sync();
+
+    // Cell types present.
+    DevBuf<int32_t> tcnt;
+    tcnt.alloc(9);
+    SC_HIP(hipMemsetAsync(tcnt.ptr, 0, tcnt.bytes(), s));
+    if (N > 0)
+    {
+        k_type_count<<<grid_for(N, BLK), BLK, 0, s>>>(static_cast<int32_t>(N), m->cltpn_b(), tcnt.ptr);
+        SC_HIP(hipGetLastError());
+    }
+    int32_t cnt[9];
+    SC_HIP(hipMemcpyAsync(cnt, tcnt.ptr, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    sync();
+    for (int t = 0; t < 9; ++t)
+    {
+        bool const ok = ndim == 2 ? (t == CT_TRI || t == CT_QUAD)
+                                  : (t == CT_TET || t == CT_HEX || t == CT_PRISM || t == CT_PYRAMID);
+        if (cnt[t] > 0 && !ok)
+        {
+            fail(SC_EVALUE, "EulerCore: cell type " + std::to_string(t) + " is not supported in "
+                                + std::to_string(ndim) + "D");
+        }
+    }
+
+    // Cell order: by type, then along the Morton curve of the centroids.
+    int_of_ref.alloc(N);
+    ref_of_int.alloc(N);
+    if (N > 0)
+    {
+        DevBuf<double> box;
+        box.alloc(6);
+        k_bbox<<<1, BLK, 0, s>>>(ndim, static_cast<int32_t>(N), m->clcnd_b(), box.ptr);
+        SC_HIP(hipGetLastError());
+        DevBuf<uint64_t> kin, kout;
+        DevBuf<int32_t> vin;
+        kin.alloc(N);
+        kout.alloc(N);
+        vin.alloc(N);
+        k_keys<<<grid_for(N, BLK), BLK, 0, s>>>(ndim, static_cast<int32_t>(N), m->cltpn_b(), m->clcnd_b(), box.ptr,
+                                                box.ptr + 3, use_sfc ? 1 : 0, kin.ptr, vin.ptr);
+        SC_HIP(hipGetLastError());
+        sort_pairs_u64(kin.ptr, kout.ptr, vin.ptr, ref_of_int.ptr, N, s);
+        k_invert<<<grid_for(N, BLK), BLK, 0, s>>>(static_cast<int32_t>(N), ref_of_int.ptr, int_of_ref.ptr);
+        SC_HIP(hipGetLastError());
+    }
+    int64_t off = 0;
+    for (int t = 0; t < 9; ++t)
+    {
+        if (cnt[t] == 0)
+        {
+            continue;
+        }
+        Group gr;
+        gr.type = t;
+        gr.off = off;
+        gr.n = cnt[t];
+        off += cnt[t];
+        groups.push_back(std::move(gr));
+    }
+    for (auto & gr : groups)
+    {
+        int nf = 0;
+        int nfield = 0;
+        switch (gr.type)
+        {
+        case CT_TRI: nf = Traits<CT_TRI>::NF; nfield = Layout<CT_TRI>::NFIELD; break;
+        case CT_QUAD: nf = Traits<CT_QUAD>::NF; nfield = Layout<CT_QUAD>::NFIELD; break;
+        case CT_TET: nf = Traits<CT_TET>::NF; nfield = Layout<CT_TET>::NFIELD; break;
+        case CT_HEX: nf = Traits<CT_HEX>::NF; nfield = Layout<CT_HEX>::NFIELD; break;
+        case CT_PRISM: nf = Traits<CT_PRISM>::NF; nfield = Layout<CT_PRISM>::NFIELD; break;
+        case CT_PYRAMID: nf = Traits<CT_PYRAMID>::NF; nfield = Layout<CT_PYRAMID>::NFIELD; break;
+        default: break;
+        }
+        gr.nbr.alloc(static_cast<int64_t>(nf) * gr.n);
+        gr.geo.alloc(static_cast<int64_t>(nfield) * gr.n);
+    }
+
+    // Solution tables (euler_solution.cpp:19-41): zero-filled.
+    cen.alloc(static_cast<int64_t>(ndim) * R);
+    for (int b = 0; b < 2; ++b)
+    {
+        s0[b].alloc(static_cast<int64_t>(neq) * R);
+        s1[b].alloc(static_cast<int64_t>(neq) * ndim * R);
+    }
+    st.alloc(static_cast<int64_t>(neq) * R);
+    stm.alloc(static_cast<int64_t>(neq) * R);
+    gam.alloc(R);
+    cflo.alloc(R);
+    cflc.alloc(R);
+    for (auto * b : {&s0[0], &s0[1], &s1[0], &s1[1], &st, &stm, &gam, &cflo, &cflc, &cen})
+    {
+        if (b->count)
+        {
+            SC_HIP(hipMemsetAsync(b->ptr, 0, b->bytes(), s));
+        }
+    }
+    // Boundary tables.
+    gkind.alloc(G);
+    ginl.alloc(G);
+    gsrc.alloc(G);
+    gmat.alloc(static_cast<int64_t>(ndim) * ndim * G);
+    fill<int32_t>(gkind.ptr, G, BC_NONE, s);
+    fill<int32_t>(ginl.ptr, G, 0, s);
+    if (G > 0)
+    {
+        if (m->nbound < G)
+        {
+            fail(SC_ERUNTIME, "EulerCore: mesh has fewer boundary faces than ghost cells");
+        }
+        k_ghost_meta<<<grid_for(G, BLK), BLK, 0, s>>>(ndim, G, m->bndfcs.ptr, m->fccls_b(), m->fcnml_b(),
+                                                      int_of_ref.ptr, gsrc.ptr, gmat.ptr);
+        SC_HIP(hipGetLastError());
+    }
+    inlet.alloc(1);
+    fill<double>(inlet.ptr, 1, 0.0, s);
+    prepare_ce();
+}
+
+void DEuler::prepare_ce()
+{
+    hipStream_t const s = stream;
+    mesh->sync();
+    MeshView mv;
+    mv.clfcs = mesh->clfcs_b();
+    mv.fccls = mesh->fccls_b();
+    mv.fcnds = mesh->fcnds_b();
+    mv.ndcrd = mesh->ndcrd_b();
+    mv.clcnd = mesh->clcnd_b();
+    mv.fccnd = mesh->fccnd_b();
+    mv.G = G;
+    mv.int_of_ref = int_of_ref.ptr;
+    mv.ref_of_int = ref_of_int.ptr;
+    if (G > 0)
+    {
+        k_cen_ghost<<<grid_for(G, BLK), BLK, 0, s>>>(ndim, G, mesh->clcnd.ptr, cen.ptr, R);
+        SC_HIP(hipGetLastError());
+    }
+    for (auto & gr : groups)
+    {
+        unsigned const nb = grid_for(gr.n, BLK);
+        switch (gr.type)
+        {
+#define SC_PREP(TT)                                                                                   \
+    case TT:                                                                                          \
+        k_prepare<TT><<<nb, BLK, 0, s>>>(mv, gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, cen.ptr, R);      \
+        SC_HIP(hipGetLastError());                                                                    \
+        k_dist<TT><<<nb, BLK, 0, s>>>(G, gr.off, gr.n, gr.geo.ptr, cen.ptr, R);                      \
+        break;
+            SC_PREP(CT_TRI)
+            SC_PREP(CT_QUAD)
+            SC_PREP(CT_TET)
+            SC_PREP(CT_HEX)
+            SC_PREP(CT_PRISM)
+            SC_PREP(CT_PYRAMID)
+#undef SC_PREP
+        default: break;
+        }
+        SC_HIP(hipGetLastError());
+    }
+    sync();
+}
+
+void DEuler::init_solution(double ga, double rho, double const * v, double p)
+{
+    // euler_solution.cpp:60-98
+    if (ga <= 1.0)
+    {
+        fail(SC_EVALUE, "EulerCore::init_solution: gamma " + std::to_string(ga) + " must be > 1");
+    }
+    if (rho <= 0.0)
+    {
+        fail(SC_EVALUE, "EulerCore::init_solution: rho " + std::to_string(rho) + " must be > 0");
+    }
+    if (p < 0.0)
+    {
+        fail(SC_EVALUE, "EulerCore::init_solution: pressure " + std::to_string(p) + " must be >= 0");
+    }
+    double vsq = 0.0;
+    for (int d = 0; d < ndim; ++d)
+    {
+        vsq += v[d] * v[d];
+    }
+    double const energy = p / (ga - 1.0) + 0.5 * rho * vsq;
+    if (R > 0)
+    {
+        k_init<<<grid_for(R, BLK), BLK, 0, stream>>>(R, G, neq, ndim, ga, rho, v[0], v[1], ndim == 3 ? v[2] : 0.0,
+                                                     energy, gam.ptr, s0[1 - cur].ptr);
+        SC_HIP(hipGetLastError());
+    }
+    sync();
+}
+
+// euler_bc.cpp:324-391
+void DEuler::add_bc(int32_t kind, std::vector<int32_t> const & faces, std::vector<double> const & value)
+{
+    int32_t const nface = mesh->nface;
+    for (int32_t f : faces)
+    {
+        if (f < 0 || f >= nface)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: face " + std::to_string(f) + " out of range [0, " + std::to_string(nface)
+                                + ")");
+        }
+    }
+    std::vector<int32_t> const jcl = mesh->fcjcl_of(faces);
+    for (size_t i = 0; i < faces.size(); ++i)
+    {
+        if (jcl[i] >= 0)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: face " + std::to_string(faces[i]) + " is not a boundary face");
+        }
+        if (-jcl[i] > G)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: face " + std::to_string(faces[i]) + " has no ghost cell");
+        }
+    }
+    if (kind == BC_INLET)
+    {
+        size_t const expect = static_cast<size_t>(ndim) + 3;
+        if (value.size() != expect)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: inlet value size " + std::to_string(value.size())
+                                + " must equal ndim+3 = " + std::to_string(expect));
+        }
+        double const rho = value[0];
+        double const pr = value[1 + ndim];
+        double const ga = value[2 + ndim];
+        if (ga <= 1.0)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: inlet gamma must be > 1");
+        }
+        if (rho <= 0.0)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: inlet rho must be > 0");
+        }
+        if (pr < 0.0)
+        {
+            fail(SC_EVALUE, "EulerCore::add_bc: inlet pressure must be >= 0");
+        }
+    }
+    else if (kind != BC_NONREFL && kind != BC_SLIPWALL)
+    {
+        fail(SC_EVALUE, "EulerCore::add_bc: unknown boundary kind");
+    }
+    EulerBoundary b;
+    b.kind = kind;
+    b.faces = faces;
+    b.value = value;
+    bcs.push_back(std::move(b));
+    refresh_bc();
+}
+
+// The ghost of a face registered more than once takes the state of its last
+// registration, exactly as the reference's in-order passes leave it.
+void DEuler::refresh_bc()
+{
+    hipStream_t const s = stream;
+    fill<int32_t>(gkind.ptr, G, BC_NONE, s);
+    fill<int32_t>(ginl.ptr, G, 0, s);
+    std::vector<double> vals;
+    int32_t ninl = 0;
+    for (auto const & b : bcs)
+    {
+        if (b.kind == BC_INLET)
+        {
+            vals.insert(vals.end(), b.value.begin(), b.value.end());
+            ++ninl;
+        }
+    }
+    inlet.alloc(std::max<int64_t>(1, static_cast<int64_t>(vals.size())));
+    if (!vals.empty())
+    {
+        SC_HIP(hipMemcpyAsync(inlet.ptr, vals.data(), sizeof(double) * vals.size(), hipMemcpyHostToDevice, s));
+    }
+    else
+    {
+        fill<double>(inlet.ptr, 1, 0.0, s);
+    }
+    int32_t iinl = 0;
+    for (auto const & b : bcs)
+    {
+        if (b.faces.empty())
+        {
+            if (b.kind == BC_INLET)
+            {
+                ++iinl;
+            }
+            continue;
+        }
+        DevBuf<int32_t> df;
+        df.alloc(static_cast<int64_t>(b.faces.size()));
+        SC_HIP(hipMemcpyAsync(df.ptr, b.faces.data(), df.bytes(), hipMemcpyHostToDevice, s));
+        int32_t const inl = b.kind == BC_INLET ? iinl : 0;
+        k_set_bc<<<grid_for(df.count, BLK), BLK, 0, s>>>(static_cast<int32_t>(df.count), df.ptr, mesh->fccls_b(), G,
+                                                         b.kind, inl, gkind.ptr, ginl.ptr);
+        SC_HIP(hipGetLastError());
+        sync();
+        if (b.kind == BC_INLET)
+        {
+            ++iinl;
+        }
+    }
+    (void)ninl;
+    sync();
+}
+
+void DEuler::table(int id, int64_t & rows, int64_t & w, int64_t & ngh, int & rank)
+{
+    ngh = G;
+    switch (id)
+    {
+    case EA_CEVOL: rows = R; w = CLMFC + 1; rank = 2; break;
+    case EA_CECND: rows = R; w = (CLMFC + 1) * ndim; rank = 2; break;
+    case EA_SFCND:
+    case EA_SFNML: rows = N; w = CLMFC * FCMND * ndim; rank = 3; ngh = 0; break;
+    case EA_SO0C:
+    case EA_SO0N:
+    case EA_SO0T:
+    case EA_STM: rows = R; w = neq; rank = 2; break;
+    case EA_SO1C:
+    case EA_SO1N: rows = R; w = neq * ndim; rank = 3; break;
+    case EA_CFLO:
+    case EA_CFLC:
+    case EA_GAMMA: rows = R; w = 1; rank = 1; break;
+    default: fail(SC_EVALUE, "EulerCore: unknown table id");
+    }
+}
+
+void DEuler::copy(int id, double * host, int64_t count, bool down)
+{
+    hipStream_t const s = stream;
+    int64_t rows, w, ngh;
+    int rank;
+    table(id, rows, w, ngh, rank);
+    if (count != rows * w)
+    {
+        fail(SC_EVALUE, "EulerCore: table size mismatch");
+    }
+    if (count == 0)
+    {
+        return;
+    }
+    DevBuf<double> api;
+    api.alloc(count);
+    if (down)
+    {
+        SC_HIP(hipMemsetAsync(api.ptr, 0, api.bytes(), s));
+    }
+    else
+    {
+        SC_HIP(hipMemcpyAsync(api.ptr, host, api.bytes(), hipMemcpyHostToDevice, s));
+    }
+    if (id == EA_CEVOL || id == EA_CECND || id == EA_SFCND || id == EA_SFNML)
+    {
+        for (auto & gr : groups)
+        {
+            unsigned const nb = grid_for(gr.n, BLK);
+            switch (gr.type)
+            {
+#define SC_GEOM(TT)                                                                                                 \
+    case TT:                                                                                                        \
+        if (down)                                                                                                   \
+        {                                                                                                           \
+            k_geom<TT, true><<<nb, BLK, 0, s>>>(id, G, gr.off, gr.n, ref_of_int.ptr, api.ptr, gr.geo.ptr, cen.ptr, R); \
+        }                                                                                                           \
+        else                                                                                                        \
+        {                                                                                                           \
+            k_geom<TT, false><<<nb, BLK, 0, s>>>(id, G, gr.off, gr.n, ref_of_int.ptr, api.ptr, gr.geo.ptr, cen.ptr, R); \
+            if (id == EA_CECND)                                                                                     \
+            {                                                                                                       \
+                k_dist<TT><<<nb, BLK, 0, s>>>(G, gr.off, gr.n, gr.geo.ptr, cen.ptr, R);                             \
+            }                                                                                                       \
+        }                                                                                                           \
+        break;
+                SC_GEOM(CT_TRI)
+                SC_GEOM(CT_QUAD)
+                SC_GEOM(CT_TET)
+                SC_GEOM(CT_HEX)
+                SC_GEOM(CT_PRISM)
+                SC_GEOM(CT_PYRAMID)
+#undef SC_GEOM
+            default: break;
+            }
+            SC_HIP(hipGetLastError());
+        }
+    }
+    else
+    {
+        double * soa = nullptr;
+        switch (id)
+        {
+        case EA_SO0C: soa = s0[cur].ptr; break;
+        case EA_SO0N: soa = s0[1 - cur].ptr; break;
+        case EA_SO0T: soa = st.ptr; break;
+        case EA_STM: soa = stm.ptr; break;
+        case EA_SO1C: soa = s1[cur].ptr; break;
+        case EA_SO1N: soa = s1[1 - cur].ptr; break;
+        case EA_CFLO: soa = cflo.ptr; break;
+        case EA_CFLC: soa = cflc.ptr; break;
+        case EA_GAMMA: soa = gam.ptr; break;
+        default: break;
+        }
+        if (down)
+        {
+            k_state<true><<<grid_for(R, BLK), BLK, 0, s>>>(R, G, static_cast<int>(w), int_of_ref.ptr, api.ptr, soa);
+        }
+        else
+        {
+            k_state<false><<<grid_for(R, BLK), BLK, 0, s>>>(R, G, static_cast<int>(w), int_of_ref.ptr, api.ptr, soa);
+        }
+        SC_HIP(hipGetLastError());
+    }
+    if (down)
+    {
+        SC_HIP(hipMemcpyAsync(host, api.ptr, api.bytes(), hipMemcpyDeviceToHost, s));
+    }
+    sync();
+}
+
+} /* end namespace scamd */
+
+using scamd::DEuler;
+using scamd::DMesh;
+using scamd::guard;
+
+extern "C"
+{
+
+int sceuler_create(void * mesh, double time_increment, int use_sfc, void ** out)
+{
+    return guard([&]() { *out = new DEuler(static_cast<DMesh *>(mesh), time_increment, use_sfc != 0); });
+}
+
+void sceuler_destroy(void * h) { delete static_cast<DEuler *>(h); }
+
+// ndim ncell ngstcell neq
+int sceuler_counts(void * h, int64_t * out)
+{
+    DEuler const & e = *static_cast<DEuler *>(h);
+    out[0] = e.ndim;
+    out[1] = e.N;
+    out[2] = e.G;
+    out[3] = e.neq;
+    return 0;
+}
+
+// which: 0 sigma0, 1 taumin, 2 tauscale, 3 time_increment
+double sceuler_get_param(void * h, int which)
+{
+    DEuler const & e = *static_cast<DEuler *>(h);
+    switch (which)
+    {
+    case 0: return e.sigma0;
+    case 1: return e.taumin;
+    case 2: return e.tauscale;
+    default: return e.dt;
+    }
+}
+
+int sceuler_set_param(void * h, int which, double v)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            switch (which)
+            {
+            case 0: e.sigma0 = v; break;
+            case 1: e.taumin = v; break;
+            case 2: e.tauscale = v; break;
+            default: scamd::fail(scamd::SC_EVALUE, "EulerCore: read-only or unknown parameter");
+            }
+        });
+}
+
+// info: rows, width (elements per row), nghost, rank
+int sceuler_table_info(void * h, int id, int64_t * info)
+{
+    return guard(
+        [&]()
+        {
+            int64_t rows, w, ngh;
+            int rank;
+            static_cast<DEuler *>(h)->table(id, rows, w, ngh, rank);
+            info[0] = rows;
+            info[1] = w;
+            info[2] = ngh;
+            info[3] = rank;
+        });
+}
+
+int sceuler_download(void * h, int id, double * host, int64_t count)
+{
+    return guard([&]() { static_cast<DEuler *>(h)->copy(id, host, count, true); });
+}
+
+int sceuler_upload(void * h, int id, double const * host, int64_t count)
+{
+    return guard([&]() { static_cast<DEuler *>(h)->copy(id, const_cast<double *>(host), count, false); });
+}
+
+// stage: 0 prepare_ce, 1 update, 2 calc_solt, 3 calc_soln, 4 calc_cfl,
+// 5 calc_dsoln, 6 bc_soln, 7 bc_dsoln, 8 march_substep, 9 clear_bc
+int sceuler_stage(void * h, int stage)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            switch (stage)
+            {
+            case 0: e.prepare_ce(); break;
+            case 1: e.update(); break;
+            case 2: e.calc_solt(); break;
+            case 3: e.run_cells<scamd::M_SOLN | scamd::M_STMEM>(); break;
+            case 4: e.run_cells<scamd::M_CFL>(); break;
+            case 5: e.run_cells<scamd::M_DSOLN | scamd::M_STMEM>(); break;
+            case 6: e.bc_soln(); break;
+            case 7: e.bc_dsoln(); break;
+            case 8: e.march_substep(); break;
+            case 9:
+                e.bcs.clear();
+                e.refresh_bc();
+                break;
+            default: scamd::fail(scamd::SC_EVALUE, "EulerCore: unknown stage");
+            }
+            e.sync();
+        });
+}
+
+int sceuler_march(void * h, int64_t steps)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            e.march(steps);
+            e.sync();
+        });
+}
+
+// Enqueue a march without waiting for it.
+int sceuler_march_async(void * h, int64_t steps)
+{
+    return guard([&]() { static_cast<DEuler *>(h)->march(steps); });
+}
+
+int sceuler_sync(void * h)
+{
+    return guard([&]() { static_cast<DEuler *>(h)->sync(); });
+}
+
+// March `steps` steps and return the average duration, in milliseconds, of
+// the fused substep kernels (all type groups) and of the whole march,
+// measured with HIP events on the solver's stream.
+int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * ms_total)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            hipEvent_t a, b, c, d;
+            SC_HIP(hipEventCreate(&a));
+            SC_HIP(hipEventCreate(&b));
+            SC_HIP(hipEventCreate(&c));
+            SC_HIP(hipEventCreate(&d));
+            float sub = 0.0f;
+            SC_HIP(hipEventRecord(c, e.stream));
+            for (int64_t it = 0; it < 2 * steps; ++it)
+            {
+                e.update();
+                SC_HIP(hipEventRecord(a, e.stream));
+                e.run_cells<scamd::M_FUSED>();
+                SC_HIP(hipEventRecord(b, e.stream));
+                SC_HIP(hipEventSynchronize(b));
+                float ms = 0.0f;
+                SC_HIP(hipEventElapsedTime(&ms, a, b));
+                sub += ms;
+            }
+            if (steps > 0)
+            {
+                e.calc_solt();
+            }
+            SC_HIP(hipEventRecord(d, e.stream));
+            SC_HIP(hipEventSynchronize(d));
+            float tot = 0.0f;
+            SC_HIP(hipEventElapsedTime(&tot, c, d));
+            *ms_substep = steps > 0 ? sub / static_cast<double>(2 * steps) : 0.0;
+            *ms_total = tot;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            (void)hipEventDestroy(c);
+            (void)hipEventDestroy(d);
+        });
+}
+
+int sceuler_init_solution(void * h, double gamma, double rho, double const * v, int nv, double p)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            if (nv != e.ndim)
+            {
+                scamd::fail(scamd::SC_EVALUE, "EulerCore::init_solution: velocity size " + std::to_string(nv)
+                                                  + " must equal ndim " + std::to_string(e.ndim));
+            }
+            double vel[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < nv; ++d)
+            {
+                vel[d] = v[d];
+            }
+            e.init_solution(gamma, rho, vel, p);
+        });
+}
+
+int sceuler_add_bc(void * h, int kind, int32_t const * faces, int64_t n, double const * value, int64_t nv)
+{
+    return guard(
+        [&]()
+        {
+            std::vector<int32_t> f(faces, faces + n);
+            std::vector<double> v(value, value + nv);
+            static_cast<DEuler *>(h)->add_bc(kind, f, v);
+        });
+}
+
+// get_normal_matrix (euler_bc.cpp:393-425) of face ifc into out[ndim*ndim].
+int sceuler_normal_matrix(void * h, int32_t ifc, double * out)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            if (ifc < 0 || ifc >= e.mesh->nface)
+            {
+                scamd::fail(scamd::SC_EINDEX, "EulerCore: face " + std::to_string(ifc) + " out of range");
+            }
+            scamd::DevBuf<double> mat;
+            mat.alloc(static_cast<int64_t>(e.ndim) * e.ndim);
+            scamd::k_face_frame<<<1, 64, 0, e.stream>>>(e.ndim, ifc, e.mesh->fcnml_b(), mat.ptr);
+            SC_HIP(hipGetLastError());
+            SC_HIP(hipMemcpyAsync(out, mat.ptr, mat.bytes(), hipMemcpyDeviceToHost, e.stream));
+            e.sync();
+        });
+}
+
+// Diagnostics: out = {sum(rho * clvol) over body cells, min cflo, max cflo}.
+int sceuler_reduce(void * h, double * out)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            constexpr int NB = 256;
+            scamd::DevBuf<double> part;
+            part.alloc(NB * 3);
+            scamd::k_reduce<<<NB, scamd::BLK, 0, e.stream>>>(e.R, e.G, e.s0[1 - e.cur].ptr, e.mesh->clvol_b(),
+                                                             e.ref_of_int.ptr, e.cflo.ptr, part.ptr);
+            SC_HIP(hipGetLastError());
+            std::vector<double> hp(NB * 3);
+            SC_HIP(hipMemcpyAsync(hp.data(), part.ptr, part.bytes(), hipMemcpyDeviceToHost, e.stream));
+            e.sync();
+            double s = 0.0;
+            double lo = DBL_MAX;
+            double hi = -DBL_MAX;
+            for (int b = 0; b < NB; ++b)
+            {
+                s += hp[b * 3];
+                lo = std::min(lo, hp[b * 3 + 1]);
+                hi = std::max(hi, hp[b * 3 + 2]);
+            }
+            out[0] = s;
+            out[1] = lo;
+            out[2] = hi;
+        });
+}
+
+} /* end extern "C" */

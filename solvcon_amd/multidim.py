@@ -1,0 +1,465 @@
+"""Multidimensional Euler CESE solver on the GPU.
+
+Drop-in for solvcon's ``EulerCore`` and ``GradientElement`` bindings
+(cpp/solvcon/multidim/pymod/wrap_multidim.cpp:17-414) and for the
+``EulerField`` views (solvcon/multidim/euler/_field.py:16-148).  State and
+geometry live in HBM; every stage runs a HIP kernel of euler.hip.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .array import GhostArray
+
+__all__ = ['EulerCore', 'GradientElement', 'EulerField']
+
+L.register({
+    'sceuler_create': (L.c_int, [L.c_vp, L.c_dbl, L.c_int, L.P_vp]),
+    'sceuler_destroy': (None, [L.c_vp]),
+    'sceuler_counts': (L.c_int, [L.c_vp, L.P_i64]),
+    'sceuler_get_param': (L.c_dbl, [L.c_vp, L.c_int]),
+    'sceuler_set_param': (L.c_int, [L.c_vp, L.c_int, L.c_dbl]),
+    'sceuler_table_info': (L.c_int, [L.c_vp, L.c_int, L.P_i64]),
+    'sceuler_download': (L.c_int, [L.c_vp, L.c_int, L.P_dbl, L.c_i64]),
+    'sceuler_upload': (L.c_int, [L.c_vp, L.c_int, L.P_dbl, L.c_i64]),
+    'sceuler_stage': (L.c_int, [L.c_vp, L.c_int]),
+    'sceuler_march': (L.c_int, [L.c_vp, L.c_i64]),
+    'sceuler_march_async': (L.c_int, [L.c_vp, L.c_i64]),
+    'sceuler_sync': (L.c_int, [L.c_vp]),
+    'sceuler_march_timed': (L.c_int, [L.c_vp, L.c_i64, L.P_dbl, L.P_dbl]),
+    'sceuler_init_solution': (L.c_int, [L.c_vp, L.c_dbl, L.c_dbl, L.P_dbl,
+                                        L.c_int, L.c_dbl]),
+    'sceuler_add_bc': (L.c_int, [L.c_vp, L.c_int, L.P_i32, L.c_i64, L.P_dbl,
+                                 L.c_i64]),
+    'sceuler_normal_matrix': (L.c_int, [L.c_vp, L.c_i32, L.P_dbl]),
+    'sceuler_reduce': (L.c_int, [L.c_vp, L.P_dbl]),
+    'scge_build': (L.c_int, [L.c_vp, L.P_dbl, L.c_i64, L.c_i64, L.c_i64,
+                             L.c_i32, L.c_dbl, L.P_i32, L.P_dbl]),
+    'scge_solve': (L.c_int, [L.c_vp, L.c_int, L.P_dbl, L.P_dbl, L.P_dbl]),
+})
+
+# Table ids of the C ABI.
+_IDS = {'cevol': 0, 'cecnd': 1, 'sfcnd': 2, 'sfnml': 3, 'so0c': 4,
+        'so0n': 5, 'so0t': 6, 'so1c': 7, 'so1n': 8, 'stm': 9, 'cflo': 10,
+        'cflc': 11, 'gamma': 12}
+# Stage ids of sceuler_stage.
+_ST = {'prepare_ce': 0, 'update': 1, 'calc_solt': 2, 'calc_soln': 3,
+       'calc_cfl': 4, 'calc_dsoln': 5, 'bc_soln': 6, 'bc_dsoln': 7,
+       'march_substep': 8, 'clear_bc': 9}
+_NONREFL, _SLIPWALL, _INLET = 0, 1, 2
+
+
+class EulerCore(object):
+    """The CESE solver of the Euler equations on a StaticMesh.
+
+    ``sfc`` (not in the reference) orders the body cells along a Morton
+    curve inside each cell-type group; results are identical either way.
+    """
+
+    SUBSTEP_RUN = 2
+
+    def __init__(self, mesh, time_increment, sfc=True):
+        L.require_gpu()
+        mesh._before()
+        h = ctypes.c_void_p()
+        L.check(L.lib().sceuler_create(mesh._h, float(time_increment),
+                                       int(bool(sfc)), ctypes.byref(h)))
+        self._h = h
+        self.mesh = mesh
+        out = (ctypes.c_int64 * 4)()
+        L.lib().sceuler_counts(self._h, out)
+        self._ndim, self._ncell, self._ngstcell, self._neq = list(out)
+        # Two physical buffers each for so0 and so1: slot *c is buffer cur.
+        self._cur = 0
+        self._mirrors = {}
+        for name in ('cevol', 'cecnd', 'sfcnd', 'sfnml', 'so0t', 'stm',
+                     'cflo', 'cflc', 'gamma'):
+            self._mirrors[name] = self._mk(name)
+        for b in (0, 1):
+            self._mirrors[('so0', b)] = self._mk(('so0', b))
+            self._mirrors[('so1', b)] = self._mk(('so1', b))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h and L._lib is not None:
+            L._lib.sceuler_destroy(h)
+            self._h = None
+
+    # -- host mirrors -------------------------------------------------------
+
+    def _tid(self, key):
+        if isinstance(key, tuple):
+            kind, b = key
+            cur = b == self._cur
+            return _IDS[kind + ('c' if cur else 'n')]
+        return _IDS[key]
+
+    def _mk(self, key):
+        info = (ctypes.c_int64 * 4)()
+        L.check(L.lib().sceuler_table_info(self._h, self._tid(key), info))
+        rows, w, ngh, rank = list(info)
+        if rank == 1:
+            shape = (rows,)
+        elif rank == 2:
+            shape = (rows, w)
+        else:
+            shape = (rows, w // self._ndim, self._ndim)
+
+        def up(a):
+            L.check(L.lib().sceuler_upload(self._h, self._tid(key), L.dptr(a),
+                                           a.size))
+
+        def down(a):
+            L.check(L.lib().sceuler_download(self._h, self._tid(key),
+                                             L.dptr(a), a.size))
+        return L.Mirror(shape, np.float64, up, down, nghost=ngh)
+
+    def _mirror(self, slot):
+        if slot in ('so0c', 'so0n', 'so1c', 'so1n'):
+            b = self._cur if slot.endswith('c') else 1 - self._cur
+            return self._mirrors[(slot[:3], b)]
+        return self._mirrors[slot]
+
+    def _before(self):
+        self.mesh._before()
+        for m in self._mirrors.values():
+            m.before()
+
+    def _after(self):
+        for m in self._mirrors.values():
+            m.after()
+
+    def _stage(self, name):
+        self._before()
+        try:
+            L.check(L.lib().sceuler_stage(self._h, _ST[name]))
+        finally:
+            self._after()
+
+    # -- information ----------------------------------------------------------
+
+    ndim = property(lambda self: self._ndim)
+    ncell = property(lambda self: self._ncell)
+    ngstcell = property(lambda self: self._ngstcell)
+    neq = property(lambda self: self._neq)
+    time_increment = property(
+        lambda self: L.lib().sceuler_get_param(self._h, 3))
+
+    def _param(i):
+        def get(self):
+            return L.lib().sceuler_get_param(self._h, i)
+
+        def put(self, v):
+            L.check(L.lib().sceuler_set_param(self._h, i, float(v)))
+        return property(get, put)
+
+    sigma0 = _param(0)
+    taumin = _param(1)
+    tauscale = _param(2)
+    del _param
+
+    # -- preparation -----------------------------------------------------------
+
+    def prepare_ce(self):
+        self._stage('prepare_ce')
+
+    def init_solution(self, gamma, rho, v, p):
+        vel = np.zeros(3)
+        v = list(v)
+        if len(v) != self._ndim:
+            raise ValueError(
+                'EulerCore::init_solution: velocity size %d must equal ndim '
+                '%d' % (len(v), self._ndim))
+        vel[:len(v)] = v
+        self._before()
+        try:
+            L.check(L.lib().sceuler_init_solution(
+                self._h, float(gamma), float(rho), L.dptr(vel), len(v),
+                float(p)))
+        finally:
+            self._after()
+
+    # -- march -----------------------------------------------------------------
+
+    def update(self):
+        L.check(L.lib().sceuler_stage(self._h, _ST['update']))
+        self._cur = 1 - self._cur
+
+    def calc_solt(self):
+        self._stage('calc_solt')
+
+    def calc_soln(self):
+        self._stage('calc_soln')
+
+    def calc_cfl(self):
+        self._stage('calc_cfl')
+
+    def calc_dsoln(self):
+        self._stage('calc_dsoln')
+
+    def march_substep(self):
+        self._before()
+        try:
+            L.check(L.lib().sceuler_stage(self._h, _ST['march_substep']))
+            self._cur = 1 - self._cur
+        finally:
+            self._after()
+
+    def march(self, steps):
+        steps = int(steps)
+        self._before()
+        try:
+            L.check(L.lib().sceuler_march(self._h, steps))
+            if steps > 0:
+                self._cur = (self._cur + 2 * steps) % 2
+        finally:
+            self._after()
+
+    def march_timed(self, steps):
+        """March and return (average fused-substep kernel time, total march
+        time) in milliseconds from HIP events on the solver's stream."""
+        self._before()
+        sub = ctypes.c_double()
+        tot = ctypes.c_double()
+        try:
+            L.check(L.lib().sceuler_march_timed(self._h, int(steps),
+                                                ctypes.byref(sub),
+                                                ctypes.byref(tot)))
+        finally:
+            self._after()
+        return sub.value, tot.value
+
+    # -- boundary conditions ------------------------------------------------------
+
+    def _add_bc(self, kind, faces, value=()):
+        f = np.ascontiguousarray(np.asarray(list(faces), dtype=np.int64)
+                                 .astype(np.int32))
+        v = np.ascontiguousarray(np.asarray(list(value), dtype=np.float64))
+        vv = v if v.size else np.zeros(1)
+        self._before()
+        L.check(L.lib().sceuler_add_bc(self._h, kind, L.iptr(f), f.size,
+                                       L.dptr(vv), v.size))
+
+    def add_nonrefl(self, faces):
+        self._add_bc(_NONREFL, faces)
+
+    def add_slipwall(self, faces):
+        self._add_bc(_SLIPWALL, faces)
+
+    def add_inlet(self, faces, value):
+        self._add_bc(_INLET, faces, value)
+
+    def clear_bc(self):
+        L.check(L.lib().sceuler_stage(self._h, _ST['clear_bc']))
+
+    def get_normal_matrix(self, ifc):
+        self._before()
+        out = np.zeros(self._ndim * self._ndim)
+        L.check(L.lib().sceuler_normal_matrix(self._h, int(ifc),
+                                              L.dptr(out)))
+        nd = self._ndim
+        return [list(out[i * nd:(i + 1) * nd]) for i in range(nd)]
+
+    def bc_soln(self):
+        self._stage('bc_soln')
+
+    def bc_dsoln(self):
+        self._stage('bc_dsoln')
+
+    # -- diagnostics ----------------------------------------------------------------
+
+    def reduce(self):
+        """(sum of so0n density times cell volume, min cflo, max cflo) over
+        the body cells, reduced on the device in FP64."""
+        self._before()
+        out = np.zeros(3)
+        L.check(L.lib().sceuler_reduce(self._h, L.dptr(out)))
+        return float(out[0]), float(out[1]), float(out[2])
+
+
+for _name in ('cevol', 'cecnd', 'sfcnd', 'sfnml', 'so0c', 'so0n', 'so0t',
+              'so1c', 'so1n', 'stm', 'cflo', 'cflc', 'gamma'):
+    setattr(EulerCore, _name,
+            property(lambda self, n=_name: GhostArray(self, n)))
+del _name
+
+
+class GradientElement(object):
+    """Gradient element of one cell (GradientElement.hpp:86-164), evaluated
+    on the GPU."""
+
+    def __init__(self, mesh, cecnd, icl, tau):
+        mesh._before()
+        arr = cecnd.ndarray if isinstance(cecnd, GhostArray) else \
+            np.asarray(cecnd)
+        ngh = cecnd.nghost if isinstance(cecnd, GhostArray) else \
+            mesh.ngstcell
+        ce = np.ascontiguousarray(arr, dtype=np.float64)
+        iout = np.zeros(34, dtype=np.int32)
+        dout = np.zeros(37)
+        L.check(L.lib().scge_build(mesh._h, L.dptr(ce), ce.shape[0],
+                                   ce.shape[1], ngh, int(icl), float(tau),
+                                   L.iptr(iout), L.dptr(dout)))
+        self._mesh = mesh
+        self._icl = int(icl)
+        self._ndim = int(iout[0])
+        self._clnfc = int(iout[1])
+        self._nfge = int(iout[2])
+        self._rcl = [int(x) for x in iout[4:4 + self._clnfc]]
+        self._faces = iout[10:34].reshape(8, 3)
+        self._ofg1 = float(dout[0])
+        self._idis = dout[1:19].reshape(6, 3)
+        self._jdis = dout[19:37].reshape(6, 3)
+
+    icl = property(lambda self: self._icl)
+    ndim = property(lambda self: self._ndim)
+    clnfc = property(lambda self: self._clnfc)
+    nfge = property(lambda self: self._nfge)
+    nfge_inverse = property(lambda self: self._ofg1)
+
+    def _ifl(self, ifl):
+        if ifl < 0 or ifl >= self._clnfc:
+            raise IndexError('GradientElement: ifl %d out of range [0, %d)'
+                             % (ifl, self._clnfc))
+
+    def _d(self, d):
+        if d < 0 or d >= self._ndim:
+            raise IndexError('GradientElement: d %d out of range [0, %d)'
+                             % (d, self._ndim))
+
+    def _ifge(self, ifge):
+        if ifge < 0 or ifge >= self._nfge:
+            raise IndexError('GradientElement: ifge %d out of range [0, %d)'
+                             % (ifge, self._nfge))
+
+    def rcl(self, ifl):
+        self._ifl(ifl)
+        return self._rcl[ifl]
+
+    def idis(self, ifl, d):
+        self._ifl(ifl)
+        self._d(d)
+        return float(self._idis[ifl, d])
+
+    def jdis(self, ifl, d):
+        self._ifl(ifl)
+        self._d(d)
+        return float(self._jdis[ifl, d])
+
+    def faces(self, ifge):
+        self._ifge(ifge)
+        return [int(x) for x in self._faces[ifge, :self._ndim]]
+
+    def displacement_matrix(self, ifge):
+        self._ifge(ifge)
+        nd = self._ndim
+        return [[float(self._idis[f - 1, d]) for d in range(nd)]
+                for f in self._faces[ifge, :nd]]
+
+    def solve_gradient(self, ifge, udf):
+        self._ifge(ifge)
+        udf = list(udf)
+        if len(udf) != self._ndim:
+            raise ValueError('GradientElement: udf size %d must equal ndim %d'
+                             % (len(udf), self._ndim))
+        dst = np.zeros(9)
+        dm = self.displacement_matrix(ifge)
+        for i, row in enumerate(dm):
+            dst[i * 3:i * 3 + len(row)] = row
+        u = np.zeros(3)
+        u[:len(udf)] = udf
+        out = np.zeros(3)
+        L.check(L.lib().scge_solve(self._mesh._h, self._ndim, L.dptr(dst),
+                                   L.dptr(u), L.dptr(out)))
+        return [float(x) for x in out[:self._ndim]]
+
+
+class EulerField(object):
+    """Body-cell views of one Euler run (solvcon/multidim/euler/_field.py:
+    16-148).  Views alias the solver's host mirrors."""
+
+    FIELDS = ('density', 'velx', 'vely', 'speed', 'pressure', 'mach',
+              'total_energy', 'cfl')
+    MOMENTA = ('momx', 'momy', 'momz')
+
+    def __init__(self, svr, mesh):
+        self.svr = svr
+        self.mesh = mesh
+
+    @property
+    def conserveds(self):
+        return (('density',) + self.MOMENTA[:self.svr.ndim]
+                + ('total_energy',))
+
+    @property
+    def conserved(self):
+        arr = self.svr.so0n
+        return arr.ndarray[arr.nghost:]
+
+    @property
+    def density(self):
+        return self.conserved[:, 0]
+
+    @property
+    def gamma(self):
+        arr = self.svr.gamma
+        return arr.ndarray[arr.nghost:]
+
+    @property
+    def centroid(self):
+        arr = self.mesh.clcnd
+        return arr.ndarray[arr.nghost:]
+
+    @property
+    def volume(self):
+        arr = self.mesh.clvol
+        return arr.ndarray[arr.nghost:]
+
+    @property
+    def vel(self):
+        cons = self.conserved
+        return cons[:, 1:1 + self.svr.ndim] / cons[:, 0][:, None]
+
+    @property
+    def velx(self):
+        return self.vel[:, 0]
+
+    @property
+    def vely(self):
+        return self.vel[:, 1]
+
+    @property
+    def speed(self):
+        return np.sqrt((self.vel ** 2).sum(axis=1))
+
+    @property
+    def total_energy(self):
+        return self.conserved[:, 1 + self.svr.ndim]
+
+    @property
+    def pressure(self):
+        return (self.gamma - 1.0) * (self.total_energy
+                                     - 0.5 * self.density * self.speed ** 2)
+
+    @property
+    def mach(self):
+        return self.speed / np.sqrt(self.gamma * self.pressure / self.density)
+
+    @property
+    def cfl(self):
+        arr = self.svr.cflo
+        return arr.ndarray[arr.nghost:]
+
+    def field(self, name):
+        if name not in self.FIELDS:
+            raise ValueError("unknown field '%s'" % name)
+        return getattr(self, name)
+
+    def calc_overall_mass(self):
+        return float(np.sum(self.density * self.volume))
+
+    def calc_cfl_range(self):
+        cfl = self.cfl
+        return float(cfl.min()), float(cfl.max())

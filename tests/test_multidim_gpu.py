@@ -1,0 +1,365 @@
+"""Multidimensional march on the GPU against the oracle, bit for bit.
+
+Covers the device mesh build (every StaticMesh table), the CE geometry,
+each stage of the fine-grained API, the fused march, boundary conditions,
+the GradientElement binding and the array semantics of the drop-in API.
+"""
+
+import numpy as np
+import pytest
+
+import meshdefs as MD
+import pair
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+SINGLES = ['triangles', 'quad', 'mixed', 'tet', 'hex', 'prism', 'pyramid']
+
+
+def _defs():
+    from solvcon_amd.gen import tri_mesh_def, mixed_mesh_def
+    return {
+        'tri16': tri_mesh_def(16, seed=2),
+        'channel': MD.quad_channel(6, 3, 4.0, 2.0),
+        'hexblk': MD.hex_block(4, 3, 3, jitter=0.2, seed=5),
+        'mixed3d': mixed_mesh_def(6, seed=3),
+    }
+
+
+def _mesh_pair(orc, defn, bcs=()):
+    return pair.gpu_mesh(defn, bcs), pair.orc_mesh(orc, defn, bcs)
+
+
+@pytest.mark.parametrize('name', SINGLES + ['tri16', 'channel', 'hexblk',
+                                            'mixed3d'])
+def test_mesh_build_bitwise(orc, name):
+    print('case: mesh build', name, flush=True)
+    defn = MD.SINGLES[name] if name in MD.SINGLES else _defs()[name]
+    g, o = _mesh_pair(orc, defn)
+    pair.same_mesh(g, o)
+
+
+def test_mesh_builder_steps(orc):
+    print('case: mesh builder steps', flush=True)
+    import solvcon_amd as sc
+    mh = sc.StaticMesh(ndim=2, nnode=4, nface=0, ncell=3)
+    mh.ndcrd[:, :] = [(0, 0), (-1, -1), (1, -1), (0, 1)]
+    mh.cltpn.fill(sc.StaticMesh.TRIANGLE)
+    mh.clnds[:, :4] = (3, 0, 1, 2), (3, 0, 2, 3), (3, 0, 3, 1)
+    mh.build_interior(do_metric=False, build_edge=False)
+    assert (mh.nface, mh.nedge) == (6, 0)
+    assert (mh.fccnd.ndarray == 0).all()
+    mh.build_interior()
+    assert (mh.nface, mh.nedge) == (6, 6)
+    np.testing.assert_almost_equal(mh.clvol, [1.0, 0.5, 0.5])
+    assert mh.nbcs == 0
+    mh.build_boundary()
+    assert mh.bndfcs.ndarray.tolist() == [[1, 0, -1], [3, 0, -1], [5, 0, -1]]
+    with pytest.raises(RuntimeError):
+        mh.build_boundary()
+    mh.build_ghost()
+    assert (mh.ngstnode, mh.ngstface, mh.ngstcell) == (3, 6, 3)
+    assert mh.clnds.nghost == 3 and mh.fcnds.shape == (12, 5)
+    # Ghost-aware indexing: row -1 is the last ghost row.
+    assert mh.fccls[1, 1] == -1
+    # Validation of node ids: a bad clnds raises instead of faulting.
+    bad = sc.StaticMesh(ndim=2, nnode=3, nface=0, ncell=1)
+    bad.cltpn[:] = sc.StaticMesh.TRIANGLE
+    bad.clnds[:, :4] = [(3, 0, 1, 7)]
+    with pytest.raises(ValueError):
+        bad.build_interior()
+
+
+def test_mesh_user_bcs(orc):
+    print('case: mesh user BCs', flush=True)
+    defn = MD.quad_channel(3, 2, 3.0, 2.0)
+    import solvcon_amd as sc
+    ndim, ndcrd, cltpn, clnds = pair.as_arrays(defn)
+    mh = sc.StaticMesh(ndim=2, nnode=ndcrd.shape[0], ncell=cltpn.shape[0])
+    mh.ndcrd[:, :] = ndcrd
+    mh.cltpn[:] = cltpn
+    mh.clnds[:, :5] = clnds
+    mh.build_interior()
+    fccls = mh.fccls.ndarray
+    bnd = [f for f in range(mh.nface) if fccls[f, 1] < 0]
+    b = mh.add_bc('left', [bnd[5], bnd[1]])
+    assert b.name == 'left'
+    mh.add_bc('right', [bnd[3]])
+    with pytest.raises(ValueError):
+        mh.add_bc('left', [bnd[0]])
+    with pytest.raises(ValueError):
+        mh.add_bc('dup', [bnd[3]])
+    with pytest.raises(ValueError):
+        mh.add_bc('interior', [f for f in range(mh.nface)
+                               if fccls[f, 1] >= 0][:1])
+    mh.build_boundary()
+    o = orc.Mesh(2, ndcrd.shape[0], 0, cltpn.shape[0])
+    o.ndcrd[:, :] = ndcrd
+    o.cltpn[:] = cltpn
+    o.clnds[:, :5] = clnds
+    o.build_interior()
+    o.add_bc('left', [bnd[5], bnd[1]])
+    o.add_bc('right', [bnd[3]])
+    o.build_boundary()
+    mh.build_ghost()
+    o.build_ghost()
+    pair.same_mesh(mh, o)
+    assert mh.bc('right').facn[:, 0].tolist() == [bnd[3]]
+    assert [x.name for x in mh.bcs] == ['left', 'right', '<NONAME>']
+
+
+def _euler_pair(orc, defn, dt=0.01, sfc=True, bcs=None):
+    import solvcon_amd as sc
+    g, o = _mesh_pair(orc, defn)
+    ge = sc.EulerCore(mesh=g, time_increment=dt, sfc=sfc)
+    oe = orc.Euler(o, dt)
+    return g, o, ge, oe
+
+
+@pytest.mark.parametrize('name', SINGLES + ['tri16', 'hexblk', 'mixed3d'])
+def test_ce_geometry_bitwise(orc, name):
+    print('case: CE geometry', name, flush=True)
+    defn = MD.SINGLES[name] if name in MD.SINGLES else _defs()[name]
+    g, o, ge, oe = _euler_pair(orc, defn)
+    pair.same_euler(ge, oe, ('cevol', 'cecnd', 'sfcnd', 'sfnml'))
+    assert (ge.ndim, ge.ncell, ge.ngstcell, ge.neq) == \
+        (oe.ndim, oe.ncell, oe.ngstcell, oe.neq)
+
+
+def _seed(ge, oe, mesh, rng):
+    """The same non-uniform, positive state in both solvers."""
+    nd, neq, g = ge.ndim, ge.neq, ge.ngstcell
+    tot = g + ge.ncell
+    s0 = np.zeros((tot, neq))
+    s0[:, 0] = rng.uniform(0.8, 1.2, tot)
+    s0[:, 1:1 + nd] = rng.uniform(-0.2, 0.2, (tot, nd))
+    s0[:, -1] = rng.uniform(2.0, 3.0, tot)
+    s1 = rng.uniform(-0.05, 0.05, (tot, neq, nd))
+    st = rng.uniform(-0.01, 0.01, (tot, neq))
+    for arr, val in (('so0c', s0), ('so0n', s0 * 1.01), ('so1c', s1),
+                     ('so1n', s1 * 0.9), ('so0t', st)):
+        getattr(ge, arr).ndarray[...] = val
+        getattr(oe, arr)[...] = val
+    ge.gamma.ndarray[...] = GAMMA
+    oe.gamma[...] = GAMMA
+    cf = rng.uniform(0.2, 0.9, tot)
+    ge.cflc.ndarray[...] = cf
+    oe.cflc[...] = cf
+
+
+@pytest.mark.parametrize('name', SINGLES + ['tri16', 'hexblk', 'mixed3d'])
+def test_stages_bitwise(orc, name):
+    print('case: stages', name, flush=True)
+    defn = MD.SINGLES[name] if name in MD.SINGLES else _defs()[name]
+    g, o, ge, oe = _euler_pair(orc, defn, dt=0.002)
+    rng = np.random.default_rng(7)
+    _seed(ge, oe, o, rng)
+    faces = list(o.bndfcs[:, 0])
+    half = len(faces) // 2
+    ge.add_nonrefl(faces[:half])
+    oe.add_bc(orc.BC_NONREFL, faces[:half])
+    ge.add_slipwall(faces[half:])
+    oe.add_bc(orc.BC_SLIPWALL, faces[half:])
+    val = [1.1] + [0.3, -0.2, 0.1][:ge.ndim] + [0.9, GAMMA]
+    ge.add_inlet(faces[:1], value=val)
+    oe.add_bc(orc.BC_INLET, faces[:1], val)
+    for stage in ('calc_solt', 'calc_soln', 'bc_soln', 'calc_cfl',
+                  'calc_dsoln', 'bc_dsoln'):
+        getattr(ge, stage)()
+        oe.call(stage)
+        pair.same_euler(ge, oe)
+    ge.update()
+    oe.call('update')
+    pair.same_euler(ge, oe)
+    ge.march_substep()
+    oe.call('march_substep')
+    pair.same_euler(ge, oe)
+
+
+def _riemann(ge, oe, o, seed):
+    from solvcon_amd.gen import riemann_box, quadrant_state
+    rho, p = riemann_box(seed)
+    g = ge.ngstcell
+    cen = o.clcnd[o.ngstcell:, :2]
+    st = quadrant_state(cen, rho, p, GAMMA)
+    if ge.ndim == 3:
+        st = np.insert(st, 3, 0.0, axis=1)
+    ge.init_solution(GAMMA, 1.0, [0.0] * ge.ndim, 1.0)
+    oe.init_solution(GAMMA, 1.0, [0.0] * ge.ndim, 1.0)
+    ge.so0n.ndarray[g:] = st
+    oe.so0n[g:] = st
+
+
+@pytest.mark.parametrize('name,sfc', [('tri16', True), ('tri16', False),
+                                      ('channel', True), ('hexblk', True),
+                                      ('mixed3d', True)])
+def test_march_bitwise(orc, name, sfc):
+    print('case: march', name, 'sfc', sfc, flush=True)
+    defn = _defs()[name]
+    g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3, sfc=sfc)
+    _riemann(ge, oe, o, seed=2)
+    faces = list(o.bndfcs[:, 0])
+    ge.add_nonrefl(faces[0::2])
+    oe.add_bc(orc.BC_NONREFL, faces[0::2])
+    ge.add_slipwall(faces[1::2])
+    oe.add_bc(orc.BC_SLIPWALL, faces[1::2])
+    for e in (ge,):
+        e.bc_soln()
+        e.bc_dsoln()
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    ge.march(12)
+    oe.march(12)
+    pair.same_euler(ge, oe)
+    # March in two calls, with a held view across them.
+    held = ge.so0n.ndarray
+    ge.march(3)
+    oe.march(3)
+    pair.same_euler(ge, oe)
+    assert np.array_equal(held, oe.so0n)
+
+
+def test_fused_equals_stages(orc):
+    print('case: fused substep vs stages', flush=True)
+    defn = _defs()['tri16']
+    import solvcon_amd as sc
+    g = pair.gpu_mesh(defn)
+    a = sc.EulerCore(mesh=g, time_increment=2e-3)
+    b = sc.EulerCore(mesh=g, time_increment=2e-3)
+    for e in (a, b):
+        e.init_solution(GAMMA, 1.0, [0.3, -0.1], 1.0)
+        e.so0n.ndarray[e.ngstcell:, 0] += np.linspace(0, 0.5, e.ncell)
+        e.add_nonrefl(list(g.bndfcs.ndarray[:, 0]))
+        e.bc_soln()
+        e.bc_dsoln()
+    for _ in range(4):
+        a.march_substep()
+        b.update()
+        b.calc_solt()
+        b.calc_soln()
+        b.bc_soln()
+        b.calc_cfl()
+        b.calc_dsoln()
+        b.bc_dsoln()
+    for name in pair.EULER_TABLES:
+        assert np.array_equal(getattr(a, name).ndarray,
+                              getattr(b, name).ndarray), name
+
+
+def test_channel_steady(orc):
+    print('case: channel steady state', flush=True)
+    import solvcon_amd as sc
+    mh = pair.gpu_mesh(MD.quad_channel(4, 2, 4.0, 2.0))
+    fccnd = mh.fccnd.ndarray[mh.fccnd.nghost:]
+    left, right, walls = [], [], []
+    for f in mh.bndfcs.ndarray[:, 0]:
+        cx = fccnd[f, 0]
+        (left if abs(cx) < 1e-9 else right if abs(cx - 4.0) < 1e-9
+         else walls).append(int(f))
+    ec = sc.EulerCore(mesh=mh, time_increment=0.04)
+    ec.init_solution(gamma=1.4, rho=1.0, v=[2.0, 0.0], p=1.0)
+    ec.add_inlet(left, value=[1.0, 2.0, 0.0, 1.0, 1.4])
+    ec.add_nonrefl(right)
+    ec.add_slipwall(walls)
+    ec.bc_soln()
+    ec.bc_dsoln()
+    ec.march(steps=5)
+    so0n = ec.so0n.ndarray
+    assert np.isfinite(so0n).all()
+    np.testing.assert_almost_equal(so0n[ec.ngstcell:, 0], 1.0, decimal=6)
+    for f in left:
+        np.testing.assert_almost_equal(ec.so0n[mh.fccls[f, 1], 0], 1.0)
+
+
+def test_api_semantics(orc):
+    print('case: EulerCore API semantics', flush=True)
+    import solvcon_amd as sc
+    defn = MD.TRIANGLES
+    mh = pair.gpu_mesh(defn)
+    ec = sc.EulerCore(mesh=mh, time_increment=0.01)
+    tot = ec.ngstcell + ec.ncell
+    assert ec.so0c.shape == (tot, 4) and ec.so1n.shape == (tot, 4, 2)
+    assert ec.cevol.shape == (tot, 7) and ec.cecnd.shape == (tot, 14)
+    assert ec.sfcnd.shape == (3, 24, 2) and ec.cflo.shape == (tot,)
+    assert ec.so0n.nghost == 3
+    np.testing.assert_almost_equal(ec.cevol[0, 1:4], [0.5, 2 / 3, 0.5])
+    assert (ec.sigma0, ec.taumin, ec.tauscale) == (3.0, 0.0, 1.0)
+    ec.sigma0, ec.taumin, ec.tauscale = 2.5, 0.1, 0.7
+    assert (ec.sigma0, ec.taumin, ec.tauscale) == (2.5, 0.1, 0.7)
+    with pytest.raises(ValueError):
+        ec.init_solution(gamma=1.4, rho=1.0, v=[0.1], p=1.0)
+    for bad in (dict(gamma=1.0), dict(rho=0.0), dict(p=-1.0)):
+        kw = dict(gamma=1.4, rho=1.2, v=[0.3, -0.15], p=0.9)
+        kw.update(bad)
+        with pytest.raises(ValueError):
+            ec.init_solution(**kw)
+    ec.init_solution(gamma=1.4, rho=1.2, v=[0.3, -0.15], p=0.9)
+    assert (ec.so0n.ndarray[:3] == 0).all()
+    # update() swaps buffers; held handles follow the swap, views do not.
+    h = ec.so0c
+    v_old = ec.so0n.ndarray
+    before = ec.so0n.ndarray.copy()
+    ec.update()
+    assert np.array_equal(h.ndarray, before)
+    assert np.array_equal(v_old, before)
+    # Boundary-condition validation.
+    with pytest.raises(ValueError):
+        ec.add_nonrefl([10 ** 7])
+    with pytest.raises(ValueError):
+        ec.add_inlet([1], value=[1.0] * 4)
+    with pytest.raises(IndexError):
+        ec.get_normal_matrix(10 ** 6)
+    m = np.array(ec.get_normal_matrix(1))
+    np.testing.assert_almost_equal(m @ m.T, np.eye(2), decimal=12)
+    # GradientElement binding.
+    ge = sc.GradientElement(mesh=mh, cecnd=ec.cecnd, icl=0, tau=1.0)
+    assert (ge.icl, ge.ndim, ge.clnfc, ge.nfge) == (0, 2, 3, 3)
+    with pytest.raises(IndexError):
+        ge.rcl(3)
+    with pytest.raises(IndexError):
+        ge.idis(0, 2)
+    with pytest.raises(IndexError):
+        ge.faces(3)
+    with pytest.raises(ValueError):
+        ge.solve_gradient(0, [0.0])
+    mh_orc = pair.orc_mesh(orc, defn)
+    oe = orc.Euler(mh_orc, 0.01)
+    og = orc.gradient_element(mh_orc, oe.cecnd, 3, 0, 1.0)
+    for ifl in range(3):
+        assert ge.rcl(ifl) == og['rcl'][ifl]
+        for d in range(2):
+            assert ge.idis(ifl, d) == og['idis'][ifl, d]
+            assert ge.jdis(ifl, d) == og['jdis'][ifl, d]
+    g = np.array([1.5, -2.7])
+    for ifge in range(3):
+        dst = np.array(ge.displacement_matrix(ifge))
+        np.testing.assert_almost_equal(ge.solve_gradient(ifge, dst @ g), g,
+                                       decimal=9)
+
+
+def test_euler_field():
+    print('case: EulerField', flush=True)
+    import solvcon_amd as sc
+    mh = pair.gpu_mesh(MD.TRIANGLES)
+    svr = sc.EulerCore(mesh=mh, time_increment=1e-2)
+    svr.init_solution(gamma=1.4, rho=1.0, v=[0.0, 0.0], p=1.0)
+    f = sc.EulerField(svr, mh)
+    assert f.conserved.shape[0] == 3
+    np.testing.assert_almost_equal(f.calc_overall_mass(), 2.0)
+    arr = svr.so0n
+    arr.ndarray[arr.nghost:, 0] = (2.0, 3.0, 4.0)
+    np.testing.assert_array_equal(f.density, (2.0, 3.0, 4.0))
+    f.conserved[:] = (1.0, 2.0, 0.0, 1.0 / 0.4 + 2.0)
+    np.testing.assert_almost_equal(f.pressure, (1.0,) * 3)
+    np.testing.assert_almost_equal(f.mach, (2.0 / np.sqrt(1.4),) * 3)
+    assert f.conserveds == ('density', 'momx', 'momy', 'total_energy')
+    with pytest.raises(ValueError):
+        f.field('nonesuch')
+    cf = svr.cflo
+    cf.ndarray[cf.nghost:] = (0.25, 1.5, 0.75)
+    assert f.calc_cfl_range() == (0.25, 1.5)
+    # The device reduction agrees with the numpy one.
+    mass, lo, hi = svr.reduce()
+    np.testing.assert_allclose(mass, f.calc_overall_mass(), rtol=1e-14)
+    assert (lo, hi) == (0.25, 1.5)
