@@ -1,0 +1,179 @@
+/*
+ * solvcon_amd: C ABI of the MI355X CESE Euler march (libsolvcon_amd.so).
+ *
+ * Plain pointers and sizes only.  Every function that can fail returns a
+ * status: 0 ok, 1 invalid argument (the reference's std::invalid_argument,
+ * ValueError in Python), 2 out of range (std::out_of_range, IndexError),
+ * 3 runtime error (std::runtime_error, RuntimeError), 4 HIP failure.  The
+ * message of the last failure on the calling thread is read with
+ * sc_last_error().  Host pointers are plain host memory; tables use the
+ * reference's row-major layout with ghost rows first.
+ *
+ * The reference binds these operations through pybind11
+ * (cpp/solvcon/onedim/pymod/wrap_onedim.cpp,
+ * cpp/solvcon/mesh/pymod/wrap_StaticMesh.cpp,
+ * cpp/solvcon/multidim/pymod/wrap_multidim.cpp); each declaration below
+ * names the reference method it replaces.
+ */
+
+#ifndef SOLVCON_AMD_H
+#define SOLVCON_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- library */
+
+/* Message of the last failure on this thread; returns its status and
+ * clears it. */
+int sc_last_error(char * buf, int n);
+/* Number of visible GPUs; 0 when there is none (never fails). */
+int sc_device_count(void);
+/* Name and architecture of the current device. */
+int sc_device_name(char * buf, int n);
+/* hipDeviceSynchronize. */
+int sc_device_sync(void);
+
+/* ------------------------------------------------------ 1D Euler (onedim) */
+/* Array ids for sc1d_*: 0 coord[n], 1 cfl[n], 2 so0[n*3], 3 so1[n*3],
+ * 4 gamma[n]. */
+
+/* Euler1DCore(ncoord, time_increment), Euler1DCore.hpp:71-75; even ncoord
+ * is rejected (Euler1DCore.cpp:25-28). */
+int sc1d_create(int64_t ncoord, double time_increment, void ** out);
+/* Euler1DCore::clone, Euler1DCore.hpp:58-62. */
+int sc1d_clone(void * h, void ** out);
+void sc1d_destroy(void * h);
+/* ncoord / time_increment, Euler1DCore.hpp:88-90. */
+int sc1d_info(void * h, int64_t * ncoord, double * time_increment);
+/* Device pointer of an array (coord/cfl/so0/so1/gamma accessors,
+ * Euler1DCore.hpp:91-104). */
+int sc1d_device_ptr(void * h, int which, double ** ptr, int64_t * count);
+/* Host <-> device copies of one array (the numpy views of
+ * wrap_onedim.cpp:89-104). */
+int sc1d_upload(void * h, int which, double const * host, int64_t count);
+int sc1d_download(void * h, int which, double * host, int64_t count);
+/* Euler1DCore::update_cfl, Euler1DCore.cpp:70-94. */
+int sc1d_update_cfl(void * h, int odd_plane);
+/* Euler1DCore::march_half_so0, Euler1DCore.cpp:96-130. */
+int sc1d_march_half_so0(void * h, int odd_plane);
+/* Euler1DCore::march_half_so1_alpha<alpha>, Euler1DCore.hpp:284-320. */
+int sc1d_march_half_so1(void * h, int alpha, int odd_plane);
+/* Euler1DCore::treat_boundary_so0/so1, Euler1DCore.cpp:132-172. */
+int sc1d_treat_boundary_so0(void * h);
+int sc1d_treat_boundary_so1(void * h);
+/* Euler1DCore::setup_march, Euler1DCore.hpp:126. */
+int sc1d_setup_march(void * h);
+/* march_half1_alpha<alpha> (second == 0) / march_half2_alpha<alpha>
+ * (second != 0), Euler1DCore.hpp:322-337. */
+int sc1d_march_half(void * h, int alpha, int second);
+/* march_alpha<alpha>(steps), Euler1DCore.hpp:339-349. */
+int sc1d_march(void * h, int alpha, int64_t steps);
+/* The same march enqueued without waiting; sc1d_sync waits. */
+int sc1d_march_async(void * h, int alpha, int64_t steps);
+int sc1d_sync(void * h);
+/* density / velocity / pressure / temperature / internal_energy / entropy
+ * (which = 0..5) into host_out[ncoord], Euler1DCore.hpp:106-168. */
+int sc1d_derived(void * h, int which, double * host_out);
+
+/* -------------------------------------------------- mesh (StaticMesh) ---- */
+/* Table ids for scmesh_*: 0 ndcrd, 1 fccnd, 2 fcnml, 3 fcara, 4 clcnd,
+ * 5 clvol, 6 fctpn, 7 cltpn, 8 clgrp, 9 fcnds, 10 fccls, 11 clnds,
+ * 12 clfcs, 13 ednds, 14 bndfcs (StaticMesh.hpp:433-451). */
+
+/* StaticMesh(ndim, nnode, nface, ncell), StaticMesh.hpp:290-311. */
+int scmesh_create(int ndim, int32_t nnode, int32_t nface, int32_t ncell, void ** out);
+void scmesh_destroy(void * h);
+/* out[10]: ndim nnode nface ncell nbound ngstnode ngstface ngstcell nedge
+ * nbcs (StaticMesh.hpp:321-332). */
+int scmesh_counts(void * h, int64_t * out);
+/* info[4]: rows (ghost + body), columns (0 for a 1-D table), ghost rows,
+ * 1 if int32 else double. */
+int scmesh_table_info(void * h, int id, int64_t * info);
+/* Device pointer of a table (its first ghost row). */
+int scmesh_table_ptr(void * h, int id, void ** ptr);
+/* Host <-> device copy of a whole table (nbytes must match). */
+int scmesh_download(void * h, int id, void * host, int64_t nbytes);
+int scmesh_upload(void * h, int id, void const * host, int64_t nbytes);
+/* StaticMesh::build_interior, StaticMesh.hpp:363-374. */
+int scmesh_build_interior(void * h, int do_metric, int do_edge);
+/* StaticMesh::build_edge, StaticMesh_interior.cpp:566-610. */
+int scmesh_build_edge(void * h);
+/* StaticMesh::build_boundary, StaticMesh_boundary.cpp:103-182. */
+int scmesh_build_boundary(void * h);
+/* StaticMesh::build_ghost, StaticMesh_boundary.cpp:185-261. */
+int scmesh_build_ghost(void * h);
+/* StaticMesh::add_bc(name, faces), StaticMesh_boundary.cpp:14-78. */
+int scmesh_add_bc(void * h, char const * name, int32_t const * faces, int64_t n);
+/* StaticMesh::bc(ibc): name and facn columns 0/1 (either may be null);
+ * *n receives the face count (StaticMesh.hpp:394-396). */
+int scmesh_bc(void * h, int ibc, char * name, int nname, int32_t * facn0, int32_t * facn1, int64_t * n);
+
+/* ------------------------------------------------- multidim (EulerCore) -- */
+/* Table ids for sceuler_*: 0 cevol, 1 cecnd, 2 sfcnd, 3 sfnml, 4 so0c,
+ * 5 so0n, 6 so0t, 7 so1c, 8 so1n, 9 stm, 10 cflo, 11 cflc, 12 gamma
+ * (euler.hpp:104-117). */
+
+/* EulerCore(mesh, time_increment), euler_ce.cpp:15-29; use_sfc orders the
+ * cells along a Morton curve inside each cell type (no reference
+ * counterpart; results are identical either way). */
+int sceuler_create(void * mesh, double time_increment, int use_sfc, void ** out);
+void sceuler_destroy(void * h);
+/* out[4]: ndim ncell ngstcell neq (euler.hpp:90-94). */
+int sceuler_counts(void * h, int64_t * out);
+/* sigma0 (0), taumin (1), tauscale (2), time_increment (3, read only),
+ * euler.hpp:95-102. */
+double sceuler_get_param(void * h, int which);
+int sceuler_set_param(void * h, int which, double v);
+/* info[4]: rows, elements per row, ghost rows, rank of the table. */
+int sceuler_table_info(void * h, int id, int64_t * info);
+/* Host <-> device copy of a whole table in the reference layout
+ * (expose_SimpleArray, cpp/solvcon/python/common.hpp:347-383). */
+int sceuler_download(void * h, int id, double * host, int64_t count);
+int sceuler_upload(void * h, int id, double const * host, int64_t count);
+/* One stage (euler.hpp:119-162): 0 prepare_ce, 1 update, 2 calc_solt,
+ * 3 calc_soln, 4 calc_cfl, 5 calc_dsoln, 6 bc_soln, 7 bc_dsoln,
+ * 8 march_substep, 9 clear_bc. */
+int sceuler_stage(void * h, int stage);
+/* EulerCore::march(steps), euler_march.cpp:350-359. */
+int sceuler_march(void * h, int64_t steps);
+/* The same march enqueued without waiting; sceuler_sync waits. */
+int sceuler_march_async(void * h, int64_t steps);
+int sceuler_sync(void * h);
+/* march(steps) with HIP-event timing: the average duration of one
+ * substep's fused kernels and the whole march, in milliseconds (no
+ * reference counterpart; used by the benchmark). */
+int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * ms_total);
+/* EulerCore::init_solution(gamma, rho, v[nv], p), euler_solution.cpp:54-99
+ * and the velocity-size check of wrap_multidim.cpp:276-301. */
+int sceuler_init_solution(void * h, double gamma, double rho, double const * v, int nv, double p);
+/* EulerCore::add_bc(kind, faces, value): kind 0 non-reflective, 1 slip
+ * wall, 2 inlet with value = [rho, v(ndim), p, gamma]
+ * (euler_bc.cpp:324-391). */
+int sceuler_add_bc(void * h, int kind, int32_t const * faces, int64_t n, double const * value, int64_t nv);
+/* EulerCore::get_normal_matrix(ifc) into out[ndim*ndim] row-major
+ * (euler_bc.cpp:393-425). */
+int sceuler_normal_matrix(void * h, int32_t ifc, double * out);
+/* FP64 device reductions over the body cells: out[3] = sum of so0n density
+ * times cell volume, min cflo, max cflo (the sums of
+ * solvcon/multidim/euler/_field.py:138-148). */
+int sceuler_reduce(void * h, double * out);
+
+/* GradientElement(mesh, cecnd, icl, tau), GradientElement.cpp:137-213,
+ * over a host cecnd table [nrow][ncol].  iout[34]: ndim, clnfc, nfge,
+ * valid, rcl[6], faces[8][3]; dout[37]: nfge_inverse, idis[6][3],
+ * jdis[6][3]. */
+int scge_build(void * mesh, double const * cecnd, int64_t nrow, int64_t ncol, int64_t nghost, int32_t icl, double tau,
+               int32_t * iout, double * dout);
+/* GradientElement::solve_gradient on a row-major dst[3][3],
+ * GradientElement.cpp:447-458. */
+int scge_solve(void * mesh, int ndim, double const * dst9, double const * udf3, double * out3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLVCON_AMD_H */

@@ -53,6 +53,12 @@ int sc_device_count(void)
     return n;
 }
 
+// Wait for all work on the current device.
+int sc_device_sync(void)
+{
+    return scamd::guard([&]() { SC_HIP(hipDeviceSynchronize()); });
+}
+
 // Name and architecture of the current device.
 int sc_device_name(char * buf, int n)
 {

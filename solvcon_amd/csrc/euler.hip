@@ -2087,46 +2087,50 @@ int sceuler_sync(void * h)
 }
 
 // March `steps` steps and return the average duration, in milliseconds, of
-// the fused substep kernels (all type groups) and of the whole march,
-// measured with HIP events on the solver's stream.
+// one substep's fused kernels (all type groups) and the duration of the
+// whole march, from HIP events recorded on the solver's stream around every
+// launch; the host waits only once, at the end.
 int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * ms_total)
 {
     return guard(
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
-            hipEvent_t a, b, c, d;
-            SC_HIP(hipEventCreate(&a));
-            SC_HIP(hipEventCreate(&b));
-            SC_HIP(hipEventCreate(&c));
-            SC_HIP(hipEventCreate(&d));
-            float sub = 0.0f;
-            SC_HIP(hipEventRecord(c, e.stream));
-            for (int64_t it = 0; it < 2 * steps; ++it)
+            int64_t const nsub = 2 * steps;
+            std::vector<hipEvent_t> ev(static_cast<size_t>(2 * nsub + 2));
+            for (auto & x : ev)
+            {
+                SC_HIP(hipEventCreate(&x));
+            }
+            SC_HIP(hipEventRecord(ev[2 * nsub], e.stream));
+            for (int64_t it = 0; it < nsub; ++it)
             {
                 e.update();
-                SC_HIP(hipEventRecord(a, e.stream));
+                SC_HIP(hipEventRecord(ev[2 * it], e.stream));
                 e.run_cells<scamd::M_FUSED>();
-                SC_HIP(hipEventRecord(b, e.stream));
-                SC_HIP(hipEventSynchronize(b));
-                float ms = 0.0f;
-                SC_HIP(hipEventElapsedTime(&ms, a, b));
-                sub += ms;
+                SC_HIP(hipEventRecord(ev[2 * it + 1], e.stream));
             }
             if (steps > 0)
             {
                 e.calc_solt();
             }
-            SC_HIP(hipEventRecord(d, e.stream));
-            SC_HIP(hipEventSynchronize(d));
+            SC_HIP(hipEventRecord(ev[2 * nsub + 1], e.stream));
+            SC_HIP(hipEventSynchronize(ev[2 * nsub + 1]));
+            double sub = 0.0;
+            for (int64_t it = 0; it < nsub; ++it)
+            {
+                float ms = 0.0f;
+                SC_HIP(hipEventElapsedTime(&ms, ev[2 * it], ev[2 * it + 1]));
+                sub += ms;
+            }
             float tot = 0.0f;
-            SC_HIP(hipEventElapsedTime(&tot, c, d));
-            *ms_substep = steps > 0 ? sub / static_cast<double>(2 * steps) : 0.0;
+            SC_HIP(hipEventElapsedTime(&tot, ev[2 * nsub], ev[2 * nsub + 1]));
+            *ms_substep = nsub > 0 ? sub / static_cast<double>(nsub) : 0.0;
             *ms_total = tot;
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            (void)hipEventDestroy(c);
-            (void)hipEventDestroy(d);
+            for (auto & x : ev)
+            {
+                (void)hipEventDestroy(x);
+            }
         });
 }
 
