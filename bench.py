@@ -97,7 +97,9 @@ def main():
     ap.add_argument('--steps', type=int, default=500)
     ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--size', type=int, default=1024)
-    ap.add_argument('--cpu-steps', type=int, default=4)
+    ap.add_argument('--cpu-steps', type=int, default=60)
+    ap.add_argument('--copy-probe', type=int, default=1024,
+                    help='MiB moved by the stream-copy probe (0: skip)')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -130,6 +132,15 @@ def main():
     achieved = B_SUBSTEP_TRI * ncell / (sub_ms * 1e-3) / 1e9
     log('bench: fused substep kernel %.4f ms avg (%d launches), %.1f GB/s '
         'algorithmic' % (sub_ms, 2 * nprof, achieved))
+    copy_gbs = None
+    if args.copy_probe > 0:
+        import ctypes
+        cms = ctypes.c_double()
+        nb = args.copy_probe * 1024 * 1024
+        _lib.check(_lib.lib().sc_stream_copy(nb, 10, ctypes.byref(cms)))
+        copy_gbs = 2 * nb / (cms.value * 1e-3) / 1e9
+        log('bench: stream copy of %d MiB: %.4f ms, %.1f GB/s (read+write)'
+            % (args.copy_probe, cms.value, copy_gbs))
     mass, cmin, cmax = ec.reduce()
     traffic = traffic_from_profiles()
     roof = {'bound': 'hbm', 'achieved': round(achieved, 1),
@@ -171,6 +182,7 @@ def main():
             'fused_substep_ms': round(sub_ms, 5),
             'b_alg_per_cell_update': 2 * B_SUBSTEP_TRI,
             'mass': mass,
+            'stream_copy_gbs': None if copy_gbs is None else round(copy_gbs, 1),
             'cfl_range': [cmin, cmax],
         },
         'roofline': roof,

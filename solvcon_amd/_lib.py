@@ -31,6 +31,7 @@ _SIGS = {
     'sc_device_count': (c_int, []),
     'sc_device_name': (c_int, [ctypes.c_char_p, c_int]),
     'sc_device_sync': (c_int, []),
+    'sc_stream_copy': (c_int, [c_i64, c_int, P_dbl]),
     # 1D core
     'sc1d_create': (c_int, [c_i64, c_dbl, P_vp]),
     'sc1d_clone': (c_int, [c_vp, P_vp]),

@@ -21,10 +21,63 @@ void set_last_error(int code, std::string const & msg)
     g_msg = msg;
 }
 
+namespace
+{
+
+// Plain device copy used to measure achievable HBM bandwidth and to
+// calibrate the HBM byte counters against a known number of bytes.
+__global__ void k_stream_copy(double2 const * __restrict__ src, double2 * __restrict__ dst, int64_t n)
+{
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        dst[i] = src[i];
+    }
+}
+
+} /* end namespace */
+
 } /* end namespace scamd */
 
 extern "C"
 {
+
+// Copy nbytes (a multiple of 16) between two device buffers reps times and
+// return the average kernel time in milliseconds.
+int sc_stream_copy(int64_t nbytes, int reps, double * ms_avg)
+{
+    return scamd::guard(
+        [&]()
+        {
+            if (nbytes <= 0 || nbytes % 16 != 0 || reps < 1)
+            {
+                scamd::fail(scamd::SC_EVALUE, "sc_stream_copy: nbytes must be a positive multiple of 16");
+            }
+            int64_t const n = nbytes / 16;
+            scamd::DevBuf<double2> a, b;
+            a.alloc(n);
+            b.alloc(n);
+            SC_HIP(hipMemset(a.ptr, 0, a.bytes()));
+            hipEvent_t e0, e1;
+            SC_HIP(hipEventCreate(&e0));
+            SC_HIP(hipEventCreate(&e1));
+            unsigned const grid = 256 * 32;
+            scamd::k_stream_copy<<<grid, 256>>>(a.ptr, b.ptr, n);
+            SC_HIP(hipGetLastError());
+            SC_HIP(hipEventRecord(e0, nullptr));
+            for (int r = 0; r < reps; ++r)
+            {
+                scamd::k_stream_copy<<<grid, 256>>>(a.ptr, b.ptr, n);
+            }
+            SC_HIP(hipEventRecord(e1, nullptr));
+            SC_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            SC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            *ms_avg = ms / reps;
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        });
+}
 
 // Copy the last error message of this thread into buf; return its status
 // code and clear it.

@@ -39,6 +39,11 @@
 #include <memory>
 #include <vector>
 
+// Minimum waves per SIMD requested for the per-cell march kernel.
+#ifndef SC_CELL_WAVES
+#define SC_CELL_WAVES 1
+#endif
+
 namespace scamd
 {
 
@@ -134,7 +139,7 @@ void fill(T * a, int64_t n, T v, hipStream_t s)
 // --------------------------------------------------------------------------
 
 template <int T, int MODE>
-__global__ void __launch_bounds__(BLK) k_cell(KP const p, GV const g)
+__global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV const g)
 {
     using TR = Traits<T>;
     using LY = Layout<T>;
