@@ -1119,24 +1119,38 @@ __global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double co
     vals[i] = i;
 }
 
-__global__ void k_bbox(int ndim, int32_t ncell, double const * clcnd, double * out)
+// Bounding box of the cell centroids in two passes: every workgroup of the
+// first pass reduces a grid-strided share into part[block][6] (lo then hi);
+// one workgroup of the second pass (nin = number of parts, clcnd null)
+// reduces the parts into out[6].
+__global__ void k_bbox(int ndim, int64_t n, double const * clcnd, double const * part_in, double * out)
 {
-    // One workgroup reduces everything: the bounding box is set-up work.
     __shared__ double lo[BLK][3];
     __shared__ double hi[BLK][3];
+    double l[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double h[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    int64_t const stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            if (clcnd != nullptr && d < ndim)
+            {
+                double const x = clcnd[i * ndim + d];
+                l[d] = fmin(l[d], x);
+                h[d] = fmax(h[d], x);
+            }
+            else if (clcnd == nullptr)
+            {
+                l[d] = fmin(l[d], part_in[i * 6 + d]);
+                h[d] = fmax(h[d], part_in[i * 6 + 3 + d]);
+            }
+        }
+    }
     for (int d = 0; d < 3; ++d)
     {
-        lo[threadIdx.x][d] = DBL_MAX;
-        hi[threadIdx.x][d] = -DBL_MAX;
-    }
-    for (int64_t i = threadIdx.x; i < ncell; i += blockDim.x)
-    {
-        for (int d = 0; d < ndim; ++d)
-        {
-            double const x = clcnd[i * ndim + d];
-            lo[threadIdx.x][d] = fmin(lo[threadIdx.x][d], x);
-            hi[threadIdx.x][d] = fmax(hi[threadIdx.x][d], x);
-        }
+        lo[threadIdx.x][d] = l[d];
+        hi[threadIdx.x][d] = h[d];
     }
     __syncthreads();
     for (unsigned s = blockDim.x / 2; s > 0; s >>= 1)
@@ -1155,19 +1169,32 @@ __global__ void k_bbox(int ndim, int32_t ncell, double const * clcnd, double * o
     {
         for (int d = 0; d < 3; ++d)
         {
-            out[d] = lo[0][d];
-            out[3 + d] = hi[0][d];
+            out[blockIdx.x * 6 + d] = lo[0][d];
+            out[blockIdx.x * 6 + 3 + d] = hi[0][d];
         }
     }
 }
 
+// Histogram of the cell types 0..8 (other values count as 0): counts
+// gather in LDS, one global atomic per workgroup and type.
 __global__ void k_type_count(int32_t ncell, int32_t const * cltpn, int32_t * cnt)
 {
-    int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncell)
+    __shared__ int32_t loc[9];
+    if (threadIdx.x < 9)
+    {
+        loc[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    int32_t const stride = gridDim.x * blockDim.x;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += stride)
     {
         int32_t const t = cltpn[i];
-        atomicAdd(&cnt[(t >= 0 && t <= 8) ? t : 0], 1);
+        atomicAdd(&loc[(t >= 0 && t <= 8) ? t : 0], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 9 && loc[threadIdx.x] != 0)
+    {
+        atomicAdd(&cnt[threadIdx.x], loc[threadIdx.x]);
     }
 }
 
@@ -1546,7 +1573,7 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
     SC_HIP(hipMemsetAsync(tcnt.ptr, 0, tcnt.bytes(), s));
     if (N > 0)
     {
-        k_type_count<<<grid_for(N, BLK), BLK, 0, s>>>(static_cast<int32_t>(N), m->cltpn_b(), tcnt.ptr);
+        k_type_count<<<std::min<int64_t>(grid_for(N, BLK), 2048), BLK, 0, s>>>(static_cast<int32_t>(N), m->cltpn_b(), tcnt.ptr);
         SC_HIP(hipGetLastError());
     }
     int32_t cnt[9];
@@ -1568,9 +1595,13 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
     ref_of_int.alloc(N);
     if (N > 0)
     {
-        DevBuf<double> box;
+        int const nparts = static_cast<int>(std::min<int64_t>(grid_for(N, BLK), 1024));
+        DevBuf<double> box, parts;
         box.alloc(6);
-        k_bbox<<<1, BLK, 0, s>>>(ndim, static_cast<int32_t>(N), m->clcnd_b(), box.ptr);
+        parts.alloc(6 * nparts);
+        k_bbox<<<nparts, BLK, 0, s>>>(ndim, N, m->clcnd_b(), nullptr, parts.ptr);
+        SC_HIP(hipGetLastError());
+        k_bbox<<<1, BLK, 0, s>>>(ndim, nparts, nullptr, parts.ptr, box.ptr);
         SC_HIP(hipGetLastError());
         DevBuf<uint64_t> kin, kout;
         DevBuf<int32_t> vin;
