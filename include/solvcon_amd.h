@@ -172,6 +172,47 @@ int scge_build(void * mesh, double const * cecnd, int64_t nrow, int64_t ncol, in
  * GradientElement.cpp:447-458. */
 int scge_solve(void * mesh, int ndim, double const * dst9, double const * udf3, double * out3);
 
+/* --------------------------------------------- spacetime scalar solvers -- */
+/* Array ids for scst_*: 0 xcoord[nx], 1 so0[nx*nvar], 2 so1[nx*nvar],
+ * 3 cfl[nx], with nx = 2*ncelm + 5 (cpp/solvcon/spacetime/core.cpp:48-98).
+ * Kernel kinds: 0 the default hooks of Kernel::reset (core.hpp:520-536),
+ * 1 LinearScalar (kernel/linear_scalar.hpp), 2 InviscidBurgers
+ * (kernel/inviscid_burgers.hpp).  The pybind11 surface replaced is
+ * cpp/solvcon/spacetime/pymod/wrap_spacetime.{hpp,cpp}. */
+
+/* Grid(xmin, xmax, nelm) coordinates into xc[nx], core.cpp:14-45 (host
+ * code; ValueError texts as the reference's). */
+int scst_grid(double xmin, double xmax, int64_t ncelm, double * xc, int64_t nx);
+/* Grid(xloc) coordinates into xc[nx], Grid::init_from_array,
+ * core.cpp:48-98. */
+int scst_grid_from(double const * xloc, int64_t n, double * xc, int64_t nx);
+/* Solver / LinearScalarSolver / InviscidBurgersSolver(grid, time_increment
+ * [, nvar]), core.hpp:740-745,1038-1056, linear_scalar.hpp:27-36 and
+ * inviscid_burgers.hpp:32-41 construct(). */
+int scst_create(int kind, double const * xc, int64_t nx, int64_t nvar, double time_increment, void ** out);
+/* SolverBase::clone (core.hpp:717-731). */
+int scst_clone(void * h, void ** out);
+void scst_destroy(void * h);
+/* time_increment setter, Field::set_time_increment (core.cpp:109-114). */
+int scst_set_time_increment(void * h, double time_increment);
+/* Whole-array copies (the so0/so1/cfl/xcoord SimpleArray views). */
+int scst_upload(void * h, int which, double const * host, int64_t count);
+int scst_download(void * h, int which, double * host, int64_t count);
+/* One pass (core.hpp:946-1024): 0 march_half_so0, 1 march_half_so1_alpha,
+ * 2 update_cfl, 3 treat_boundary_so0, 4 treat_boundary_so1,
+ * 5 march_half1_alpha, 6 march_half2_alpha; alpha 0, 1 or 2. */
+int scst_stage(void * h, int stage, int alpha, int odd_plane);
+/* march_alpha<alpha>(steps), core.hpp:1026-1036. */
+int scst_march(void * h, int alpha, int64_t steps);
+int scst_march_async(void * h, int alpha, int64_t steps);
+int scst_sync(void * h);
+/* One element value at xindex (Selm::xn/xp/tn/tp/so0p/update_cfl,
+ * Celm::calc_so0/calc_so1_alpha, core.hpp:458-672): what 0 xn, 1 xp, 2 tn,
+ * 3 tp, 4 so0p, 5 cfl (store != 0 writes it), 6 calc_so0, 7 calc_so1. */
+int scst_eval(void * h, int what, int alpha, int64_t xindex, int64_t iv, int store, double * out);
+/* get_so0p(iv, odd_plane) into host_out[nselm - odd], core.hpp:812-824. */
+int scst_so0p(void * h, int64_t iv, int odd_plane, double * host_out, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

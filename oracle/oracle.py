@@ -87,6 +87,18 @@ def lib():
         L.orcge_det.restype = ctypes.c_double
         L.orcge_det.argtypes = [ctypes.c_int, dp]
         L.orcge_adj.argtypes = [ctypes.c_int, dp, dp]
+        st = [ctypes.c_int64, ctypes.c_int64, ctypes.c_double, dp, dp, dp,
+              dp]
+        L.orcst_grid.restype = ctypes.c_int
+        L.orcst_grid.argtypes = [ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int64, dp]
+        L.orcst_grid_from.restype = ctypes.c_int
+        L.orcst_grid_from.argtypes = [dp, ctypes.c_int64, dp]
+        L.orcst_stage.argtypes = [ctypes.c_int] * 4 + st
+        L.orcst_march.argtypes = [ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int64] + st
+        L.orcst_eval.restype = ctypes.c_double
+        L.orcst_eval.argtypes = [ctypes.c_int] * 3 + [ctypes.c_int64] * 2 + st
         _lib = L
     return _lib
 
@@ -443,3 +455,84 @@ def ge_adj(ndim, a):
     out = np.zeros(9)
     lib().orcge_adj(ndim, _dp(m), _dp(out))
     return out.reshape(3, 3)[:ndim, :ndim].copy()
+
+
+# --------------------------------------------------------- spacetime -------
+
+ST_DEFAULT, ST_LINEAR, ST_BURGERS = 0, 1, 2
+
+
+def st_grid(xmin=None, xmax=None, ncelm=None, xloc=None):
+    """Coordinates of spacetime Grid(xmin, xmax, ncelm) or Grid(xloc)
+    (cpp/solvcon/spacetime/core.cpp:14-98); ValueError on invalid
+    arguments."""
+    if xloc is not None:
+        xloc = np.ascontiguousarray(xloc, dtype=np.float64)
+        xc = np.zeros(max(2 * (xloc.size - 1) + 5, 0))
+        rc = lib().orcst_grid_from(_dp(xloc), xloc.size, _dp(xc))
+    else:
+        xc = np.zeros(max(2 * int(ncelm) + 5, 0))
+        rc = lib().orcst_grid(xmin, xmax, int(ncelm), _dp(xc))
+    if rc != 0:
+        raise ValueError('invalid grid')
+    return xc
+
+
+class StSolver:
+    """Restated spacetime SolverBase (core.hpp:706-1036) over host arrays:
+    kind 0 the default kernel hooks, 1 LinearScalar, 2 InviscidBurgers."""
+
+    def __init__(self, kind, xc, time_increment, nvar=1):
+        self.kind = kind
+        self.xc = np.array(xc, dtype=np.float64)
+        self.nx = self.xc.size
+        self.ncelm = (self.nx - 5) // 2
+        self.nselm = self.ncelm + 1
+        self.nvar = nvar
+        self.time_increment = time_increment
+        self.so0 = np.zeros((self.nx, nvar))
+        self.so1 = np.zeros((self.nx, nvar))
+        self.cfl = np.zeros(self.nx)
+
+    def clone(self):
+        o = StSolver(self.kind, self.xc, self.time_increment, self.nvar)
+        o.so0[...] = self.so0
+        o.so1[...] = self.so1
+        o.cfl[...] = self.cfl
+        return o
+
+    def _args(self):
+        return (self.nx, self.nvar, self.time_increment, _dp(self.xc),
+                _dp(self.so0), _dp(self.so1), _dp(self.cfl))
+
+    def stage(self, stage, alpha=2, odd=False):
+        lib().orcst_stage(self.kind, stage, alpha, int(bool(odd)),
+                          *self._args())
+
+    def march_half_so0(self, odd):
+        self.stage(0, odd=odd)
+
+    def march_half_so1(self, alpha, odd):
+        self.stage(1, alpha, odd)
+
+    def update_cfl(self, odd):
+        self.stage(2, odd=odd)
+
+    def treat_boundary_so0(self):
+        self.stage(3)
+
+    def treat_boundary_so1(self):
+        self.stage(4)
+
+    def setup_march(self):
+        self.update_cfl(False)
+
+    def march(self, alpha, steps):
+        lib().orcst_march(self.kind, alpha, steps, *self._args())
+
+    def sel(self, odd):
+        """xindex of the solution elements of one plane."""
+        return 2 + 2 * np.arange(self.nselm - int(bool(odd))) + int(bool(odd))
+
+    def eval(self, what, y, iv=0, alpha=2):
+        return lib().orcst_eval(self.kind, what, alpha, y, iv, *self._args())

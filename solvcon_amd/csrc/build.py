@@ -15,7 +15,7 @@ PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, 'libsolvcon_amd.so')
 OBJDIR = os.path.join(HERE, '_build')
 
-SOURCES = ['capi.hip', 'onedim.hip', 'mesh.hip', 'euler.hip', 'order.hip', 'ge.hip']
+SOURCES = ['capi.hip', 'onedim.hip', 'mesh.hip', 'euler.hip', 'order.hip', 'ge.hip', 'spacetime.hip']
 
 FLAGS = [
     '--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
