@@ -1,15 +1,30 @@
 """Benchmark of the CESE Euler march on one MI355X.
 
-Default workload: the 2D Euler CESE march on the jittered triangle mesh of
-the unit square (1024 x 1024 quads split into 2,097,152 triangles, interior
-nodes jittered by up to 0.2h, seed 2), a four-quadrant Riemann box with
-rho and p drawn from U(0.1, 1.5), u = v = 0, gamma = 1.4, non-reflective
-boundaries and dt set so the largest initial CFL number is 0.8.  A step is
-one CESE step (two substeps) over the whole mesh; value is cell-updates/s
-with the state resident in HBM.
+Default workload (``--config 2d``): the 2D Euler CESE march on the jittered
+triangle mesh of the unit square.
+- Mesh: 1024 x 1024 quads split into 2,097,152 triangles; interior nodes
+  jittered by up to 0.2h; seed 2.
+- Initial state: a four-quadrant Riemann box with rho and p drawn from
+  U(0.1, 1.5), u = v = 0, gamma = 1.4.
+- Non-reflective boundaries; dt set so that the largest initial CFL number
+  is 0.8.
+A step is one CESE step (two substeps) over the whole mesh. ``value`` is
+cell-updates/s with the state resident in HBM.
+
+Other modes:
+- ``--config 3d``: the mixed hex/tet/prism/pyramid block of the unit cube
+  (200^3 hexes, 30% tetrahedral and 20% prismatic layers). It carries a
+  spherical blast (rho = 1, p = 10 inside r = 0.1, p = 0.1 outside), slip
+  walls on x = 0, y = 0 and z = 0, and non-reflective boundaries elsewhere.
+  It runs 200 steps.
+- ``--config 1d``: the Sod tube, 2001 points, 600 steps of march_alpha2.
+- ``--batched``: thousands of independent 2D solves in one batched march,
+  against the plain form.
+- ``--served``: a Poisson request stream of 1D/2D/3D solves through the
+  request loop, with p50/p99 latency, against serial plain execution.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K]
-[--warmup W] [--size M] [--no-cpu]
+[--warmup W] [--config 2d|3d|1d] [--batched] [--served] [--no-cpu]
 """
 
 import argparse
@@ -24,47 +39,40 @@ sys.path.insert(0, ROOT)
 METRIC = ('cell-updates/s FP64 Euler CESE; batched solves/s + p99 latency '
           'vs plain')
 PEAK_HBM_GBS = 8000.0
-# Algorithmic bytes per cell per substep of a triangle at packed widths:
-# state in 8q + 8qd + 8 (gamma), state out 8q + 8qd + 16 (cflo, cflc),
-# CE geometry 8(f+1) + 8d(f+1) + 16 S d, neighbour list 4(f+1);
-# q = 4, d = 2, f = 3, S = 6.
-B_SUBSTEP_TRI = (8 * 4 + 8 * 8 + 8) + (8 * 4 + 8 * 8 + 16) \
-    + (8 * 4 + 8 * 2 * 4 + 16 * 6 * 2) + 4 * 4
+
+# Cell type ids (solvcon's StaticMesh) -> (ndim, faces, sub-faces).
+CELL_SHAPE = {3: (2, 4, 8), 4: (2, 3, 6), 5: (3, 6, 24), 6: (3, 4, 12),
+              7: (3, 5, 18), 8: (3, 5, 16)}
+CELL_NAME = {3: 'quad', 4: 'triangle', 5: 'hexahedron', 6: 'tetrahedron',
+             7: 'prism', 8: 'pyramid'}
+
+
+def b_substep(ctype):
+    """Algorithmic bytes per cell and substep at packed widths:
+    - state in: so0c 8q, so1c 8qd, gamma 8;
+    - state out: so0n 8q, so1n 8qd, cflo and cflc 16;
+    - CE geometry: volumes 8(f+1), BCE centroids 8d(f+1), sub-face
+      centroids and normals 16 S d;
+    - neighbour list 4(f+1).
+    Here q = d + 2."""
+    d, f, s = CELL_SHAPE[ctype]
+    q = d + 2
+    state = (8 * q + 8 * q * d + 8) + (8 * q + 8 * q * d + 16)
+    geom = 8 * (f + 1) + 8 * d * (f + 1) + 16 * s * d + 4 * (f + 1)
+    return state + geom
+
+
+B_SUBSTEP_TRI = b_substep(4)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup_2d(m, seed=2, cfl=0.8):
-    import numpy as np
-    import solvcon_amd as sc
-    from solvcon_amd.gen import (tri_mesh_def, riemann_box, quadrant_state,
-                                 build_static_mesh, cfl_dt)
-    defn = tri_mesh_def(m, seed=seed)
-    mh = build_static_mesh(defn)
-    ncell = mh.ncell
-    ngst = mh.ngstcell
-    clcnd = mh.clcnd.ndarray[ngst:].copy()
-    rho, p = riemann_box(seed)
-    state = quadrant_state(clcnd, rho, p, 1.4)
-    # dt from the CFL rule needs the CE geometry: a provisional core gives it.
-    probe = sc.EulerCore(mesh=mh, time_increment=1.0)
-    dt = cfl_dt(probe.cecnd.ndarray, ncell, ngst, 2, 3, state, 1.4, cfl)
-    del probe
-    ec = sc.EulerCore(mesh=mh, time_increment=dt)
-    ec.init_solution(1.4, 1.0, [0.0, 0.0], 1.0)
-    ec.so0n.ndarray[ngst:] = state
-    ec.add_nonrefl(mh.bndfcs.ndarray[:, 0])
-    ec.bc_soln()
-    ec.bc_dsoln()
-    return defn, mh, ec, state, dt
-
-
-def traffic_from_profiles():
-    """HBM bytes per fused launch from the committed rocprofv3 counter
+def traffic_from_profiles(name):
+    """HBM bytes per fused launch from a committed rocprofv3 counter
     summary, when one exists."""
-    path = os.path.join(ROOT, 'profiles', 'hbm_traffic_tri2d.md')
+    path = os.path.join(ROOT, 'profiles', name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -73,15 +81,83 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(defn, state, dt, steps, threads):
-    """The oracle (the CPU restatement of the reference march) on the same
-    mesh and state, timed over a few steps."""
+def workload_2d(m, seed=2):
+    from solvcon_amd.gen import tri_mesh_def, riemann_box, quadrant_state
+    defn = tri_mesh_def(m, seed=seed)
+    rho, p = riemann_box(seed)
+
+    def state(cen):
+        return quadrant_state(cen, rho, p, 1.4)
+
+    def bcs(mh):
+        return [('nonrefl', mh.bndfcs.ndarray[:, 0].copy())]
+    desc = ('2D Euler CESE march, jittered triangle mesh of the unit square '
+            '(%dx%d quads x 2 = %d cells), Riemann box, non-reflective BCs, '
+            'CFL 0.8' % (m, m, 2 * m * m))
+    return defn, state, bcs, desc
+
+
+def workload_3d(n, seed=3):
+    import numpy as np
+    from solvcon_amd.gen import mixed_mesh_def, blast_state
+    defn = mixed_mesh_def(n, seed=seed)
+
+    def state(cen):
+        return blast_state(cen, 1.4)
+
+    def bcs(mh):
+        bf = mh.bndfcs.ndarray[:, 0].copy()
+        c = mh.fccnd.ndarray[mh.ngstface + bf]
+        wall = (np.abs(c) < 1e-12).any(axis=1)
+        return [('slipwall', bf[wall]), ('nonrefl', bf[~wall])]
+    desc = ('3D Euler CESE march, mixed hex/tet/prism/pyramid block of the '
+            'unit cube (%d^3 hexes, 30%% tetrahedral and 20%% prismatic '
+            'layers, %d cells), spherical blast, slip walls on x=0/y=0/z=0, '
+            'non-reflective elsewhere, CFL 0.8' % (n, len(defn[2])))
+    return defn, state, bcs, desc
+
+
+def setup_mesh_core(defn, state, bcs, cfl=0.8):
+    """Device mesh, EulerCore, initial state, BCs and the CFL-rule dt."""
+    import solvcon_amd as sc
+    from solvcon_amd.gen import build_static_mesh
+    mh = build_static_mesh(defn)
+    ngst = mh.ngstcell
+    ec = sc.EulerCore(mesh=mh, time_increment=1.0)
+    ec.init_solution(1.4, 1.0, [0.0] * mh.ndim, 1.0)
+    st = state(mh.clcnd.ndarray[ngst:])
+    ec.so0n.ndarray[ngst:] = st
+    for kind, faces in bcs(mh):
+        getattr(ec, 'add_' + kind)(faces)
+    ec.bc_soln()
+    ec.bc_dsoln()
+    dt = ec.cfl_dt(cfl)
+    ec.set_time_increment(dt)
+    return mh, ec, st, dt
+
+
+class _View(object):
+    def __init__(self, a):
+        self.ndarray = a
+
+
+def cpu_baseline_mesh(defn, state, bcs_of, dt, steps, threads):
+    """The oracle (the CPU restatement of the reference march) on the given
+    mesh, state and boundaries, timed over a few steps."""
+    import numpy as np
     from oracle import oracle as O
     m = O.make_mesh(*defn)
     oe = O.Euler(m, dt, nthreads=threads)
-    oe.init_solution(1.4, 1.0, [0.0, 0.0], 1.0)
-    oe.so0n[oe.ngstcell:] = state
-    oe.add_bc(O.BC_NONREFL, m.bndfcs[:, 0])
+    oe.init_solution(1.4, 1.0, [0.0] * m.ndim, 1.0)
+    ngst = oe.ngstcell
+    oe.so0n[ngst:] = state(m.clcnd[ngst:])
+    mv = type('MeshView', (), {})()
+    mv.bndfcs = _View(m.bndfcs)
+    mv.fccnd = _View(m.fccnd)
+    mv.ngstface = m.ngstface
+    kinds = {'nonrefl': O.BC_NONREFL, 'slipwall': O.BC_SLIPWALL}
+    for kind, faces in bcs_of(mv):
+        oe.add_bc(kinds[kind], np.asarray(faces))
     oe.call('bc_soln')
     oe.call('bc_dsoln')
     oe.march(1)
@@ -91,30 +167,27 @@ def cpu_baseline(defn, state, dt, steps, threads):
     return oe.ncell * steps / el, el
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=500)
-    ap.add_argument('--warmup', type=int, default=20)
-    ap.add_argument('--size', type=int, default=1024)
-    ap.add_argument('--cpu-steps', type=int, default=60)
-    ap.add_argument('--copy-probe', type=int, default=1024,
-                    help='MiB moved by the stream-copy probe (0: skip)')
-    ap.add_argument('--no-cpu', action='store_true')
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print('bench.py: this build measures one GPU; --gpus %d is not '
-              'supported (every GPU box here has one GPU)' % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
+def group_mix(ec):
+    """Body cells per cell type of an EulerCore, from its mesh."""
+    import numpy as np
+    cl = ec.mesh.cltpn.ndarray[ec.mesh.ngstcell:]
+    t, c = np.unique(cl, return_counts=True)
+    return {int(a): int(b) for a, b in zip(t, c)}
 
-    import solvcon_amd as sc
+
+def run_mesh(args, which):
     from solvcon_amd import _lib
-    log('bench: building the %dx%dx2 triangle mesh on the GPU' %
-        (args.size, args.size))
+    if which == '2d':
+        defn, state, bcs, desc = workload_2d(args.size)
+        traffic_name = 'hbm_traffic_tri2d.md'
+    else:
+        defn, state, bcs, desc = workload_3d(args.size3)
+        traffic_name = 'hbm_traffic_mixed3d.md'
+    log('bench: %s mesh of %d cells generated' % (which, len(defn[2])))
     t0 = time.perf_counter()
-    defn, mh, ec, state, dt = setup_2d(args.size)
-    log('bench: set-up %.1f s, %d cells, dt %.6g' %
+    mh, ec, st, dt = setup_mesh_core(defn, state, bcs)
+    _lib.check(_lib.lib().sc_device_sync())
+    log('bench: device set-up %.1f s, %d cells, dt %.6g' %
         (time.perf_counter() - t0, ec.ncell, dt))
     ec.march(args.warmup)
     _lib.check(_lib.lib().sc_device_sync())
@@ -126,11 +199,13 @@ def main():
     value = ncell * args.steps / el
     log('bench: %d steps in %.3f s, %.4g cell-updates/s' %
         (args.steps, el, value))
-    # Kernel time per launch from HIP events on the solver's stream.
+    # Kernel time per substep from HIP events on the solver's stream.
     nprof = max(10, min(args.steps, 100))
     sub_ms, tot_ms = ec.march_timed(nprof)
-    achieved = B_SUBSTEP_TRI * ncell / (sub_ms * 1e-3) / 1e9
-    log('bench: fused substep kernel %.4f ms avg (%d launches), %.1f GB/s '
+    mix = group_mix(ec)
+    b_sub = sum(b_substep(t) * n for t, n in mix.items())  # bytes/substep
+    achieved = b_sub / (sub_ms * 1e-3) / 1e9
+    log('bench: fused substep kernels %.4f ms avg (%d substeps), %.1f GB/s '
         'algorithmic' % (sub_ms, 2 * nprof, achieved))
     copy_gbs = None
     if args.copy_probe > 0:
@@ -142,23 +217,29 @@ def main():
         log('bench: stream copy of %d MiB: %.4f ms, %.1f GB/s (read+write)'
             % (args.copy_probe, cms.value, copy_gbs))
     mass, cmin, cmax = ec.reduce()
-    traffic = traffic_from_profiles()
     roof = {'bound': 'hbm', 'achieved': round(achieved, 1),
             'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
             'frac': round(achieved / PEAK_HBM_GBS, 4),
-            'traffic': traffic}
+            'traffic': traffic_from_profiles(traffic_name)}
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get('OMP_NUM_THREADS', '0') or 0) or \
             min(os.cpu_count() or 1, 16)
-        log('bench: timing the oracle on %d threads, %d steps' %
-            (threads, args.cpu_steps))
-        cval, cel = cpu_baseline(defn, state, dt, args.cpu_steps, threads)
+        if which == '2d':
+            cdefn, csteps = defn, args.cpu_steps
+            sample = '%d steps of the same %d-cell mesh and state'
+        else:
+            # The full 3D mesh takes minutes to build on the CPU: the
+            # sample is the same generator and state at a smaller size.
+            cdefn = workload_3d(args.cpu_size3)[0]
+            csteps = args.cpu_steps3
+            sample = '%d steps of the same generator and state at %d cells'
+        log('bench: timing the oracle on %d threads' % threads)
+        cval, cel = cpu_baseline_mesh(cdefn, state, bcs, dt, csteps, threads)
         cpu = {'value': round(cval, 1), 'unit': 'cell-updates/s',
                'cores': threads, 'kind': 'port',
-               'sample': '%d steps of the same %d-cell mesh and state '
-                         '(%.1f s), OpenMP over cells' %
-                         (args.cpu_steps, ncell, cel)}
+               'sample': (sample % (csteps, len(cdefn[2])))
+               + ' (%.1f s), OpenMP over cells' % cel}
     out = {
         'metric': METRIC,
         'value': round(value, 1),
@@ -173,14 +254,12 @@ def main():
         'dtype': 'f64',
         'data': 'synthetic',
         'config': {
-            'workload': '2D Euler CESE march, jittered triangle mesh of the '
-                        'unit square (%dx%d quads x 2 = %d cells), '
-                        'Riemann box, non-reflective BCs, CFL 0.8'
-                        % (args.size, args.size, ncell),
+            'workload': desc,
             'cells': ncell,
+            'cell_mix': {CELL_NAME[t]: n for t, n in mix.items()},
             'dt': dt,
             'fused_substep_ms': round(sub_ms, 5),
-            'b_alg_per_cell_update': 2 * B_SUBSTEP_TRI,
+            'b_alg_per_cell_update': round(2 * b_sub / ncell, 1),
             'mass': mass,
             'stream_copy_gbs': None if copy_gbs is None else round(copy_gbs, 1),
             'cfl_range': [cmin, cmax],
@@ -189,6 +268,260 @@ def main():
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def run_1d(args):
+    """Sod tube, 2001 points, 600 steps of march_alpha2 per run
+    (cell-updates per run = 600 x (2001 - 4))."""
+    from solvcon_amd.onedim import ShockTube
+    from solvcon_amd import _lib
+    ncoord, steps = 2001, 600
+    st = ShockTube()
+    st.build_constant(gamma=1.4, pressure1=1.0, density1=1.0,
+                      pressure5=0.1, density5=0.125)
+    st.build_numerical(xmin=-1, xmax=1, ncoord=ncoord,
+                       time_increment=0.4 / steps)
+    base = st.svr
+    runs = []
+    for _ in range(args.warmup + args.steps):
+        s = base.clone()
+        _lib.check(_lib.lib().sc_device_sync())
+        t0 = time.perf_counter()
+        s.march_alpha2(steps)
+        runs.append(time.perf_counter() - t0)
+    runs = sorted(runs[args.warmup:])
+    med = runs[len(runs) // 2]
+    upd = steps * (ncoord - 4)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ts = []
+        for _ in range(5):
+            o = O.sod_setup(ncoord=ncoord, time_increment=0.4 / steps)
+            t0 = time.perf_counter()
+            o.march(2, steps)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        cpu = {'value': round(upd / ts[2], 1), 'unit': 'cell-updates/s',
+               'cores': 1, 'kind': 'port',
+               'sample': 'the whole Sod run (2001 points, 600 steps), '
+                         'median of 5'}
+    out = {
+        'metric': 'cell-updates/s FP64 1D Euler CESE (Sod tube)',
+        'value': round(upd / med, 1), 'unit': 'cell-updates/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_run': round(med * 1e3, 4), 'higher_is_better': True,
+        'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': 'Sod shock tube, 2001 points, 600 steps of '
+                               'march_alpha2 per run; one run per timed step',
+                   'run_ms_min': round(runs[0] * 1e3, 4),
+                   'run_ms_max': round(runs[-1] * 1e3, 4)},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args):
+    """The batched form: ``--batch-n`` independent 2D triangle solves.
+    - Mesh sizes are log-uniform in [1e3, 2e5] cells.
+    - Step counts are uniform in [100, 1000].
+    - Each instance has its own seeded Riemann box.
+    All instances run in one batched march. The plain form is timed on a
+    sample of the same instances solved one after another."""
+    import numpy as np
+    from solvcon_amd import _lib
+    from solvcon_amd.batch import (batch_config, config_defs, config_states,
+                                   BatchedEuler, build_mesh)
+    from solvcon_amd import EulerCore
+    m, mseed, rseed, steps = batch_config(ninst=args.batch_n)
+    t0 = time.perf_counter()
+    defs = config_defs(m, mseed)
+    states = config_states(rseed)
+    t1 = time.perf_counter()
+    b = BatchedEuler(defs, states, steps)
+    _lib.check(_lib.lib().sc_device_sync())
+    t2 = time.perf_counter()
+    log('bench: batch of %d instances, %d cells, %.3g cell-updates; '
+        'generate %.1f s, device set-up %.1f s' %
+        (b.ninst, b.coff[-1], b.cell_updates, t1 - t0, t2 - t1))
+    t3 = time.perf_counter()
+    dev_ms = b.march()
+    t4 = time.perf_counter()
+    wall = t4 - t3
+    log('bench: batched march %.3f s wall (%.1f ms device), %.4g '
+        'cell-updates/s, %.1f solves/s' %
+        (wall, dev_ms, b.cell_updates / wall, b.ninst / wall))
+    # The plain form on a sample spread over the sizes: each instance gets
+    # its own mesh and EulerCore and marches alone.
+    order = np.argsort(b.ncells)
+    pick = order[np.linspace(0, b.ninst - 1, args.plain_sample).astype(int)]
+    tm = []
+    for k in pick:
+        defn = defs[k]
+        ta = time.perf_counter()
+        mh = build_mesh(defn)
+        ec = EulerCore(mh, float(b.dts[k]))
+        ec.init_solution(1.4, 1.0, [0.0, 0.0], 1.0)
+        cen = mh.clcnd.ndarray[mh.ngstcell:]
+        ec.so0n.ndarray[mh.ngstcell:] = states(cen, k)
+        ec.add_nonrefl(mh.bndfcs.ndarray[:, 0])
+        ec.bc_soln()
+        ec.bc_dsoln()
+        tb = time.perf_counter()
+        ec.march(int(steps[k]))
+        tc = time.perf_counter()
+        same = np.array_equal(ec.so0n.ndarray[mh.ngstcell:],
+                              b.body('so0n', k))
+        tm.append((b.ncells[k], steps[k], tb - ta, tc - tb, same))
+    tm = np.array(tm, dtype=float)
+    # Serial plain time of the whole batch from a fit of the sample:
+    # march time = a * steps + c * cells * steps (launch cost plus work).
+    A = np.stack([tm[:, 1], tm[:, 0] * tm[:, 1]], axis=1)
+    coef, *_ = np.linalg.lstsq(A, tm[:, 3], rcond=None)
+    plain_march = float(coef[0] * steps.sum()
+                        + coef[1] * (b.ncells * steps).sum())
+    setup_fit = np.polyfit(tm[:, 0], tm[:, 2], 1)
+    plain_setup = float(np.polyval(setup_fit, b.ncells).sum())
+    log('bench: plain sample of %d: march fit %.3g s/step + %.3g s per '
+        'cell-update; whole batch serially %.1f s march + %.1f s set-up; '
+        'all equal: %s' % (len(pick), coef[0], coef[1], plain_march,
+                           plain_setup, bool(tm[:, 4].all())))
+    out = {
+        'metric': 'batched solves/s (FP64 Euler CESE, 2D triangles)',
+        'value': round(b.ninst / wall, 2),
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'higher_is_better': True,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {
+            'workload': '%d independent 2D jittered-triangle Euler solves, '
+                        'sizes log-uniform 1e3-2e5 cells, 100-1000 steps, '
+                        'Riemann boxes, CFL 0.8' % b.ninst,
+            'cells': int(b.coff[-1]),
+            'cell_updates': b.cell_updates,
+            'batched_wall_s': round(wall, 4),
+            'batched_device_ms': round(dev_ms, 2),
+            'batched_cell_updates_per_s': round(b.cell_updates / wall, 1),
+            'batched_setup_s': round(t2 - t1, 2),
+            'plain_serial_march_s_est': round(plain_march, 2),
+            'plain_serial_setup_s_est': round(plain_setup, 2),
+            'plain_sample': int(len(pick)),
+            'sample_equal_bitwise': bool(tm[:, 4].all()),
+        },
+        'vs_plain': round(plain_march / wall, 3),
+        'vs_plain_end_to_end': round((plain_march + plain_setup)
+                                     / (wall + t2 - t1), 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_served(args):
+    """The serving form: ``--serve-n`` requests of 1D/2D/3D solves in a
+    50/35/15 mix.
+    - Sizes are log-uniform in 1e3-2e6 cells; steps are uniform in
+      100-1000.
+    - Arrivals follow a Poisson process at several loads, given as
+      multiples of the serial plain capacity.
+    Each load is served by the request loop, and the same stream is
+    modelled as a FIFO queue over the measured serial plain service
+    times."""
+    import numpy as np
+    from solvcon_amd.serve import (make_requests, payload, serve,
+                                   serve_serial, fifo_latency)
+    base = make_requests(args.serve_n, rate=1.0, seed=5)
+    t0 = time.perf_counter()
+    payloads = [payload(r) for r in base]
+    log('bench: %d request payloads generated in %.1f s (client side)' %
+        (len(base), time.perf_counter() - t0))
+    sj, svc = serve_serial(base, payloads)
+    mu = 1.0 / svc.mean()
+    log('bench: serial plain service %.4f s mean, capacity %.2f solves/s' %
+        (svc.mean(), mu))
+    loads = []
+    equal = True
+    for frac in args.serve_loads:
+        reqs = make_requests(args.serve_n, rate=frac * mu, seed=5)
+        jobs, wall = serve(reqs, payloads,
+                           budget_bytes=int(args.serve_budget_gb * 2 ** 30))
+        lat = np.array([j.t_end - j.req.arrival for j in jobs])
+        span = max(j.t_end for j in jobs)
+        arr = np.array([r.arrival for r in reqs])
+        slat, sspan = fifo_latency(arr, svc)
+        for a, b in zip(jobs, sj):
+            equal = equal and np.array_equal(a.result, b.result)
+        row = {'load': frac, 'rate': round(frac * mu, 3),
+               'solves_s': round(len(jobs) / span, 3),
+               'p50_s': round(float(np.percentile(lat, 50)), 4),
+               'p99_s': round(float(np.percentile(lat, 99)), 4),
+               'serial_solves_s': round(len(jobs) / sspan, 3),
+               'serial_p50_s': round(float(np.percentile(slat, 50)), 4),
+               'serial_p99_s': round(float(np.percentile(slat, 99)), 4)}
+        loads.append(row)
+        log('bench: load %.2f: %s' % (frac, row))
+    top = loads[-1]
+    out = {
+        'metric': 'served solves/s and p99 latency (FP64 Euler CESE, '
+                  '1D/2D/3D mix)',
+        'value': top['solves_s'], 'unit': 'solves/s', 'n_gpus': 1,
+        'higher_is_better': True, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': '%d requests, 50/35/15 1D/2D/3D, sizes '
+                               'log-uniform 1e3-2e6 cells, 100-1000 steps, '
+                               'Poisson arrivals' % args.serve_n,
+                   'serial_service_mean_s': round(float(svc.mean()), 4),
+                   'serial_capacity_solves_s': round(mu, 3),
+                   'budget_gb': args.serve_budget_gb,
+                   'results_equal_serial_bitwise': bool(equal),
+                   'loads': loads},
+        'p99_s': top['p99_s'],
+        'vs_plain': round(top['solves_s'] / top['serial_solves_s'], 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=None)
+    ap.add_argument('--warmup', type=int, default=None)
+    ap.add_argument('--config', choices=('2d', '3d', '1d'), default='2d')
+    ap.add_argument('--size', type=int, default=1024)
+    ap.add_argument('--size3', type=int, default=200)
+    ap.add_argument('--cpu-steps', type=int, default=60)
+    ap.add_argument('--cpu-size3', type=int, default=60)
+    ap.add_argument('--cpu-steps3', type=int, default=4)
+    ap.add_argument('--copy-probe', type=int, default=1024,
+                    help='MiB moved by the stream-copy probe (0: skip)')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--batched', action='store_true',
+                    help='measure the batched form instead')
+    ap.add_argument('--batch-n', type=int, default=4096)
+    ap.add_argument('--plain-sample', type=int, default=24)
+    ap.add_argument('--served', action='store_true',
+                    help='measure the serving form instead')
+    ap.add_argument('--serve-n', type=int, default=60)
+    ap.add_argument('--serve-loads', type=float, nargs='+',
+                    default=[0.5, 1.0, 2.0, 4.0])
+    ap.add_argument('--serve-budget-gb', type=float, default=256.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU; --gpus %d is not '
+              'supported (every GPU box here has one GPU)' % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    defaults = {'2d': (500, 20), '3d': (200, 5), '1d': (21, 3)}[args.config]
+    if args.steps is None:
+        args.steps = defaults[0]
+    if args.warmup is None:
+        args.warmup = defaults[1]
+    if args.batched:
+        run_batched(args)
+    elif args.served:
+        run_served(args)
+    elif args.config == '1d':
+        run_1d(args)
+    else:
+        run_mesh(args, args.config)
 
 
 if __name__ == '__main__':

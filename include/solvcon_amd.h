@@ -36,6 +36,10 @@ int sc_device_count(void);
 int sc_device_name(char * buf, int n);
 /* hipDeviceSynchronize. */
 int sc_device_sync(void);
+/* Stream-copy probe: copies nbytes between two device buffers reps times
+ * and returns the average kernel time in ms (no reference counterpart; it
+ * calibrates the HBM counters and the achievable bandwidth). */
+int sc_stream_copy(int64_t nbytes, int reps, double * ms_avg);
 
 /* ------------------------------------------------------ 1D Euler (onedim) */
 /* Array ids for sc1d_*: 0 coord[n], 1 cfl[n], 2 so0[n*3], 3 so1[n*3],
@@ -75,6 +79,9 @@ int sc1d_march(void * h, int alpha, int64_t steps);
 /* The same march enqueued without waiting; sc1d_sync waits. */
 int sc1d_march_async(void * h, int alpha, int64_t steps);
 int sc1d_sync(void * h);
+/* *done = 1 when the handle's stream is idle, else 0 (no reference
+ * counterpart; for request loops that poll many solves). */
+int sc1d_query(void * h, int * done);
 /* density / velocity / pressure / temperature / internal_energy / entropy
  * (which = 0..5) into host_out[ncoord], Euler1DCore.hpp:106-168. */
 int sc1d_derived(void * h, int which, double * host_out);
@@ -124,8 +131,9 @@ int sceuler_create(void * mesh, double time_increment, int use_sfc, void ** out)
 void sceuler_destroy(void * h);
 /* out[4]: ndim ncell ngstcell neq (euler.hpp:90-94). */
 int sceuler_counts(void * h, int64_t * out);
-/* sigma0 (0), taumin (1), tauscale (2), time_increment (3, read only),
- * euler.hpp:95-102. */
+/* sigma0 (0), taumin (1), tauscale (2), time_increment (3),
+ * euler.hpp:95-102; the reference fixes time_increment at construction,
+ * setting it here (not on a batched solver) has no counterpart. */
 double sceuler_get_param(void * h, int which);
 int sceuler_set_param(void * h, int which, double v);
 /* info[4]: rows, elements per row, ghost rows, rank of the table. */
@@ -143,6 +151,28 @@ int sceuler_march(void * h, int64_t steps);
 /* The same march enqueued without waiting; sceuler_sync waits. */
 int sceuler_march_async(void * h, int64_t steps);
 int sceuler_sync(void * h);
+/* *done = 1 when the handle's stream is idle, else 0 (no reference
+ * counterpart). */
+int sceuler_query(void * h, int * done);
+/* dt that puts the largest calc_cfl number of the current so0n at cfl,
+ * reduced on the device; sceuler_set_param(h, 3, dt) then applies it (no
+ * reference counterpart: the reference takes dt from its caller). */
+int sceuler_cfl_dt(void * h, double cfl, double * dt);
+/* Batched solves (no reference counterpart; the per-instance march is
+ * EulerCore::march, euler_march.cpp:339-359): an EulerCore over a
+ * disjoint-union mesh of ninst instances.  inst_of_cell[ncell] is the
+ * instance of every body cell, steps[ninst] the step count of every
+ * instance. */
+int sceuler_create_batch(void * mesh, int64_t ninst, int32_t const * inst_of_cell, int64_t const * steps, int use_sfc,
+                         void ** out);
+/* Time increment of every instance (by instance id). */
+int sceuler_batch_set_dt(void * h, double const * dt);
+/* Set and return in dt_out[ninst] the dt of every instance that puts its
+ * largest initial calc_cfl number at cfl. */
+int sceuler_batch_cfl_dt(void * h, double cfl, double * dt_out);
+/* March every instance its own step count; *ms_total (if not null)
+ * receives the device time of the march. */
+int sceuler_march_batch(void * h, double * ms_total);
 /* march(steps) with HIP-event timing: the average duration of one
  * substep's fused kernels and the whole march, in milliseconds (no
  * reference counterpart; used by the benchmark). */

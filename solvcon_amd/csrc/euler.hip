@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cstring>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -79,6 +80,7 @@ enum Mode : int
     M_CFL = 4,
     M_DSOLN = 8,
     M_STMEM = 16,
+    M_BATCH = 32,
     M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
 };
 
@@ -104,14 +106,20 @@ struct KP
     int32_t const * ginl;
     double const * gmat;
     double const * inlet;
+    // Batched solves: the instance rank of every body cell (internal order)
+    // and the half/quarter time increments of every rank.
+    int32_t const * inst;
+    double const * bhdt;
+    double const * bqdt;
 };
 
 struct GV
 {
     int64_t off;
-    int64_t n;
+    int64_t n;   // cells of the group: the stride of its SoA tables
     int32_t const * nbr;
     double const * geo;
+    int64_t cnt; // cells to process, a prefix of the group
 };
 
 template <typename T>
@@ -149,7 +157,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
     constexpr int NFGE = TR::NFGE;
 
     int64_t const i = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (i >= g.n)
+    if (i >= g.cnt)
     {
         return;
     }
@@ -157,6 +165,14 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
     int64_t const R = p.R;
     int64_t const row = p.G + g.off + i;
     double const * geo = g.geo + i;
+    double hdt = p.hdt;
+    double qdt = p.qdt;
+    if constexpr ((MODE & M_BATCH) != 0)
+    {
+        int32_t const r = p.inst[g.off + i];
+        hdt = p.bhdt[r];
+        qdt = p.bqdt[r];
+    }
 
     int64_t nb[NF];
 #pragma unroll
@@ -237,7 +253,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    usfc[k] = p.qdt * st[f][k];
+                    usfc[k] = qdt * st[f][k];
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
                     {
@@ -259,7 +275,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
                         }
                         dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
                     }
-                    acc[k] -= p.hdt * dot;
+                    acc[k] -= hdt * dot;
                 }
             }
         }
@@ -355,7 +371,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
         double const pr = ga1 * (energy - ke);
         double const pra = (pr + fabs(pr)) / 2.0;
         double const wspd = sqrt(ga * pra / density) + sqrt(momsq) / density;
-        double const co = p.hdt * wspd / dist;
+        double const co = hdt * wspd / dist;
         p.cflo[row] = co;
         cflc_v = (co - 1.0) * pra / (pra + TINYJ) + 1.0;
         p.cflc[row] = cflc_v;
@@ -498,7 +514,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                double val = p.s0c[k * R + j] + p.hdt * st[f][k] - u0[k];
+                double val = p.s0c[k * R + j] + hdt * st[f][k] - u0[k];
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
@@ -1091,8 +1107,11 @@ __device__ inline uint64_t spread3(uint64_t x)
     return x;
 }
 
+// Sort keys of the body cells: the cell type in the top 4 bits, then (for
+// batched solves) the instance rank in the next 20, then the Morton code of
+// the centroid.
 __global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double const * clcnd, double const * lo,
-                       double const * hi, int use_sfc, uint64_t * keys, int32_t * vals)
+                       double const * hi, int use_sfc, int32_t const * rank, uint64_t * keys, int32_t * vals)
 {
     int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ncell)
@@ -1102,7 +1121,7 @@ __global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double co
     uint64_t code = 0;
     if (use_sfc)
     {
-        int const bits = ndim == 2 ? 30 : 20;
+        int const bits = rank != nullptr ? (ndim == 2 ? 20 : 13) : (ndim == 2 ? 30 : 20);
         double const scale = static_cast<double>((1ULL << bits) - 1);
         uint64_t q[3] = {0, 0, 0};
         for (int d = 0; d < ndim; ++d)
@@ -1114,6 +1133,10 @@ __global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double co
         }
         code = ndim == 2 ? (spread2(q[0]) | (spread2(q[1]) << 1))
                          : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+    }
+    if (rank != nullptr)
+    {
+        code = (static_cast<uint64_t>(rank[i] & 0xfffff) << 40) | (code & 0xffffffffffULL);
     }
     keys[i] = (static_cast<uint64_t>(cltpn[i] & 0xf) << 60) | (code & 0x0fffffffffffffffULL);
     vals[i] = i;
@@ -1354,6 +1377,65 @@ __global__ void k_reduce(int64_t R, int64_t G, double const * s0n, double const 
     }
 }
 
+// Instance rank of every body cell in internal order, and the histogram
+// of (cell type, rank) that gives each type group its per-rank prefixes.
+__global__ void k_batch_inst(int32_t n, int32_t const * ref_of_int, int32_t const * rank_ref,
+                             int32_t const * cltpn, int32_t ninst, int32_t * inst, int32_t * hist)
+{
+    int32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        int32_t const c = ref_of_int[i];
+        int32_t const r = rank_ref[c];
+        inst[i] = r;
+        atomicAdd(&hist[static_cast<int64_t>(cltpn[c] & 0xf) * ninst + r], 1);
+    }
+}
+
+// Largest calc_cfl number of every instance (of the whole mesh when inst is
+// null) at time increment 1 (hdt 0.5) from the so0n state (euler_solution.cpp:114-159), as the bit pattern of a
+// non-negative double so that an integer maximum orders it.
+template <int T>
+__global__ void k_batch_cflmax(int64_t R, int64_t G, int64_t off, int64_t n, double const * geo,
+                               double const * s0n, double const * gam, int32_t const * inst,
+                               unsigned long long * cmax)
+{
+    using LY = Layout<T>;
+    constexpr int ND = Traits<T>::ND;
+    constexpr int NEQ = ND + 2;
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+    {
+        return;
+    }
+    int64_t const row = G + off + i;
+    double u0[NEQ];
+#pragma unroll
+    for (int k = 0; k < NEQ; ++k)
+    {
+        u0[k] = s0n[k * R + row];
+    }
+    double const dist = geo[LY::FD * n + i];
+    double const ga = gam[row];
+    double const density = u0[0];
+    double momsq = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+    {
+        momsq += u0[1 + d] * u0[1 + d];
+    }
+    double const ke = momsq / (2.0 * density);
+    double const pr = (ga - 1.0) * (u0[NEQ - 1] - ke);
+    double const pra = (pr + fabs(pr)) / 2.0;
+    double const wspd = sqrt(ga * pra / density) + sqrt(momsq) / density;
+    double const v = 0.5 * wspd / dist;
+    if (v >= 0.0)
+    {
+        atomicMax(&cmax[inst != nullptr ? inst[off + i] : 0],
+                  static_cast<unsigned long long>(__double_as_longlong(v)));
+    }
+}
+
 } /* end namespace */
 
 struct EulerBoundary
@@ -1395,7 +1477,19 @@ struct DEuler
     DevBuf<int32_t> gkind, ginl, gsrc;
     DevBuf<double> gmat, inlet;
 
-    DEuler(DMesh * m, double time_increment, bool sfc);
+    // Batched solves of independent instances in one disjoint-union mesh:
+    // ranks order the instances by descending step count, cells are sorted
+    // by (type, rank, Morton), so the cells still marching at any step are
+    // a prefix of every type group (bcum[group][rank]).
+    int64_t ninst = 0;
+    std::vector<int64_t> bsteps;       // by rank, non-increasing
+    std::vector<int32_t> brank_of;     // instance id -> rank
+    std::vector<std::vector<int64_t>> bcum;
+    DevBuf<int32_t> brank_ref, binst;
+    DevBuf<double> bhdt, bqdt;
+
+    DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch = 0, int32_t const * inst_of_cell = nullptr,
+           int64_t const * steps = nullptr);
     ~DEuler()
     {
         if (stream != nullptr)
@@ -1431,6 +1525,9 @@ struct DEuler
         p.ginl = ginl.ptr;
         p.gmat = gmat.ptr;
         p.inlet = inlet.ptr;
+        p.inst = binst.ptr;
+        p.bhdt = bhdt.ptr;
+        p.bqdt = bqdt.ptr;
         return p;
     }
 
@@ -1444,7 +1541,7 @@ struct DEuler
             {
                 continue;
             }
-            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr};
+            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, gr.n};
             unsigned const nb = grid_for(gr.n, BLK);
             switch (gr.type)
             {
@@ -1459,6 +1556,66 @@ struct DEuler
             SC_HIP(hipGetLastError());
         }
     }
+
+    // One fused substep of the batched march over the cells of the first
+    // nact ranks of every type group.
+    void run_cells_batch(int64_t nact)
+    {
+        KP const p = params();
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+        {
+            auto & gr = groups[gi];
+            int64_t const cnt = bcum[gi][nact];
+            if (cnt == 0)
+            {
+                continue;
+            }
+            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, cnt};
+            unsigned const nb = grid_for(cnt, BLK);
+            constexpr int MB = M_FUSED | M_BATCH;
+            switch (gr.type)
+            {
+            case CT_TRI: k_cell<CT_TRI, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_QUAD: k_cell<CT_QUAD, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TET: k_cell<CT_TET, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_HEX: k_cell<CT_HEX, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PRISM: k_cell<CT_PRISM, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PYRAMID: k_cell<CT_PYRAMID, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            default: fail(SC_ERUNTIME, "EulerCore: unsupported cell type in group");
+            }
+            SC_HIP(hipGetLastError());
+        }
+    }
+
+    // Every instance marches its own step count: two fused substeps per
+    // step over the ranks still marching, then calc_solt once.  A finished
+    // instance keeps its state in place: the buffer swap of the others
+    // returns to the same parity after each whole step.
+    void march_batch()
+    {
+        int64_t const maxs = ninst > 0 ? bsteps[0] : 0;
+        int64_t nact = ninst;
+        for (int64_t step = 0; step < maxs; ++step)
+        {
+            while (nact > 0 && bsteps[nact - 1] <= step)
+            {
+                --nact;
+            }
+            for (int sub = 0; sub < 2; ++sub)
+            {
+                update();
+                run_cells_batch(nact);
+            }
+        }
+        if (maxs > 0)
+        {
+            calc_solt();
+        }
+    }
+
+    void batch_set_dt(double const * dt_inst);
+    void batch_cfl_dt(double cfl, double * dt_inst);
+    double cfl_max();
 
     void calc_solt()
     {
@@ -1549,10 +1706,12 @@ struct DEuler
     void copy(int id, double * host, int64_t count, bool down);
 };
 
-DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
+DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32_t const * inst_of_cell,
+               int64_t const * steps)
     : mesh(m)
     , dt(time_increment)
     , use_sfc(sfc)
+    , ninst(nbatch)
 {
     if (m->ndim != 2 && m->ndim != 3)
     {
@@ -1590,6 +1749,46 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
         }
     }
 
+    // Batched instances: rank by descending step count (stable in the
+    // instance id), rank of every cell in the reference order.
+    if (ninst > 0)
+    {
+        if (ninst > (1 << 20))
+        {
+            fail(SC_EVALUE, "EulerCore batch: at most 1048576 instances");
+        }
+        std::vector<int32_t> order(static_cast<size_t>(ninst));
+        for (int64_t k = 0; k < ninst; ++k)
+        {
+            if (steps[k] < 1)
+            {
+                fail(SC_EVALUE, "EulerCore batch: every instance needs at least one step");
+            }
+            order[k] = static_cast<int32_t>(k);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return steps[a] > steps[b]; });
+        brank_of.assign(static_cast<size_t>(ninst), 0);
+        bsteps.assign(static_cast<size_t>(ninst), 0);
+        for (int64_t r = 0; r < ninst; ++r)
+        {
+            brank_of[order[r]] = static_cast<int32_t>(r);
+            bsteps[r] = steps[order[r]];
+        }
+        std::vector<int32_t> rank_ref(static_cast<size_t>(N));
+        for (int64_t c = 0; c < N; ++c)
+        {
+            int32_t const k = inst_of_cell[c];
+            if (k < 0 || k >= ninst)
+            {
+                fail(SC_EVALUE, "EulerCore batch: instance id of cell " + std::to_string(c) + " out of range");
+            }
+            rank_ref[c] = brank_of[k];
+        }
+        brank_ref.alloc(N);
+        SC_HIP(hipMemcpyAsync(brank_ref.ptr, rank_ref.data(), brank_ref.bytes(), hipMemcpyHostToDevice, s));
+        sync();
+    }
+
     // Cell order: by type, then along the Morton curve of the centroids.
     int_of_ref.alloc(N);
     ref_of_int.alloc(N);
@@ -1609,7 +1808,7 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
         kout.alloc(N);
         vin.alloc(N);
         k_keys<<<grid_for(N, BLK), BLK, 0, s>>>(ndim, static_cast<int32_t>(N), m->cltpn_b(), m->clcnd_b(), box.ptr,
-                                                box.ptr + 3, use_sfc ? 1 : 0, kin.ptr, vin.ptr);
+                                                box.ptr + 3, use_sfc ? 1 : 0, brank_ref.ptr, kin.ptr, vin.ptr);
         SC_HIP(hipGetLastError());
         sort_pairs_u64(kin.ptr, kout.ptr, vin.ptr, ref_of_int.ptr, N, s);
         k_invert<<<grid_for(N, BLK), BLK, 0, s>>>(static_cast<int32_t>(N), ref_of_int.ptr, int_of_ref.ptr);
@@ -1628,6 +1827,36 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
         gr.n = cnt[t];
         off += cnt[t];
         groups.push_back(std::move(gr));
+    }
+    if (ninst > 0)
+    {
+        binst.alloc(N);
+        DevBuf<int32_t> hist;
+        hist.alloc(9 * ninst);
+        SC_HIP(hipMemsetAsync(hist.ptr, 0, hist.bytes(), s));
+        if (N > 0)
+        {
+            k_batch_inst<<<grid_for(N, BLK), BLK, 0, s>>>(static_cast<int32_t>(N), ref_of_int.ptr, brank_ref.ptr,
+                                                           m->cltpn_b(), static_cast<int32_t>(ninst), binst.ptr,
+                                                           hist.ptr);
+            SC_HIP(hipGetLastError());
+        }
+        std::vector<int32_t> h(static_cast<size_t>(9 * ninst));
+        SC_HIP(hipMemcpyAsync(h.data(), hist.ptr, hist.bytes(), hipMemcpyDeviceToHost, s));
+        sync();
+        for (auto & gr : groups)
+        {
+            std::vector<int64_t> cum(static_cast<size_t>(ninst + 1), 0);
+            for (int64_t r = 0; r < ninst; ++r)
+            {
+                cum[r + 1] = cum[r] + h[static_cast<size_t>(gr.type * ninst + r)];
+            }
+            bcum.push_back(std::move(cum));
+        }
+        bhdt.alloc(ninst);
+        bqdt.alloc(ninst);
+        std::vector<double> d(static_cast<size_t>(ninst), time_increment);
+        batch_set_dt(d.data());
     }
     for (auto & gr : groups)
     {
@@ -1686,6 +1915,101 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc)
     inlet.alloc(1);
     fill<double>(inlet.ptr, 1, 0.0, s);
     prepare_ce();
+}
+
+// Time increments of the batched instances, given per instance id; the
+// kernels read them per rank (hdt = dt/2, qdt = dt/4 as for one solve).
+void DEuler::batch_set_dt(double const * dt_inst)
+{
+    std::vector<double> h(static_cast<size_t>(ninst));
+    std::vector<double> q(static_cast<size_t>(ninst));
+    for (int64_t k = 0; k < ninst; ++k)
+    {
+        h[brank_of[k]] = dt_inst[k] * 0.5;
+        q[brank_of[k]] = dt_inst[k] * 0.25;
+    }
+    SC_HIP(hipMemcpyAsync(bhdt.ptr, h.data(), bhdt.bytes(), hipMemcpyHostToDevice, stream));
+    SC_HIP(hipMemcpyAsync(bqdt.ptr, q.data(), bqdt.bytes(), hipMemcpyHostToDevice, stream));
+    sync();
+}
+
+// dt of every instance such that its largest initial calc_cfl number is cfl
+// (from the current so0n), then set.
+void DEuler::batch_cfl_dt(double cfl, double * dt_inst)
+{
+    DevBuf<unsigned long long> cm;
+    cm.alloc(ninst);
+    SC_HIP(hipMemsetAsync(cm.ptr, 0, cm.bytes(), stream));
+    double const * s0n = s0[1 - cur].ptr;
+    for (auto & gr : groups)
+    {
+        unsigned const nb = grid_for(gr.n, BLK);
+        switch (gr.type)
+        {
+#define SC_CM(TT)                                                                                              \
+    case TT:                                                                                                   \
+        k_batch_cflmax<TT><<<nb, BLK, 0, stream>>>(R, G, gr.off, gr.n, gr.geo.ptr, s0n, gam.ptr, binst.ptr, cm.ptr); \
+        break;
+            SC_CM(CT_TRI)
+            SC_CM(CT_QUAD)
+            SC_CM(CT_TET)
+            SC_CM(CT_HEX)
+            SC_CM(CT_PRISM)
+            SC_CM(CT_PYRAMID)
+#undef SC_CM
+        default: break;
+        }
+        SC_HIP(hipGetLastError());
+    }
+    std::vector<unsigned long long> h(static_cast<size_t>(ninst));
+    SC_HIP(hipMemcpyAsync(h.data(), cm.ptr, cm.bytes(), hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int64_t k = 0; k < ninst; ++k)
+    {
+        double v;
+        std::memcpy(&v, &h[brank_of[k]], sizeof(v));
+        if (!(v > 0.0))
+        {
+            fail(SC_EVALUE, "EulerCore batch: instance " + std::to_string(k) + " has no positive wave speed");
+        }
+        dt_inst[k] = cfl / v;
+    }
+    batch_set_dt(dt_inst);
+}
+
+// Largest calc_cfl number over the body cells at dt = 1, from so0n.
+double DEuler::cfl_max()
+{
+    DevBuf<unsigned long long> cm;
+    cm.alloc(1);
+    SC_HIP(hipMemsetAsync(cm.ptr, 0, cm.bytes(), stream));
+    double const * s0n = s0[1 - cur].ptr;
+    for (auto & gr : groups)
+    {
+        unsigned const nb = grid_for(gr.n, BLK);
+        switch (gr.type)
+        {
+#define SC_CM(TT)                                                                                              \
+    case TT:                                                                                                   \
+        k_batch_cflmax<TT><<<nb, BLK, 0, stream>>>(R, G, gr.off, gr.n, gr.geo.ptr, s0n, gam.ptr, nullptr, cm.ptr); \
+        break;
+            SC_CM(CT_TRI)
+            SC_CM(CT_QUAD)
+            SC_CM(CT_TET)
+            SC_CM(CT_HEX)
+            SC_CM(CT_PRISM)
+            SC_CM(CT_PYRAMID)
+#undef SC_CM
+        default: break;
+        }
+        SC_HIP(hipGetLastError());
+    }
+    unsigned long long h = 0;
+    SC_HIP(hipMemcpyAsync(&h, cm.ptr, sizeof(h), hipMemcpyDeviceToHost, stream));
+    sync();
+    double v;
+    std::memcpy(&v, &h, sizeof(v));
+    return v;
 }
 
 void DEuler::prepare_ce()
@@ -2003,6 +2327,103 @@ int sceuler_create(void * mesh, double time_increment, int use_sfc, void ** out)
     return guard([&]() { *out = new DEuler(static_cast<DMesh *>(mesh), time_increment, use_sfc != 0); });
 }
 
+// Batched EulerCore over a disjoint-union mesh of ninst instances:
+// inst_of_cell[ncell] gives the instance of every body cell (reference
+// order), steps[ninst] the step count of every instance.
+int sceuler_create_batch(void * mesh, int64_t ninst, int32_t const * inst_of_cell, int64_t const * steps, int use_sfc,
+                         void ** out)
+{
+    return guard(
+        [&]()
+        {
+            if (ninst < 1)
+            {
+                scamd::fail(scamd::SC_EVALUE, "EulerCore batch: ninst must be at least 1");
+            }
+            *out = new DEuler(static_cast<DMesh *>(mesh), 1.0, use_sfc != 0, ninst, inst_of_cell, steps);
+        });
+}
+
+int sceuler_batch_set_dt(void * h, double const * dt)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst == 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
+            }
+            e.batch_set_dt(dt);
+        });
+}
+
+int sceuler_batch_cfl_dt(void * h, double cfl, double * dt_out)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst == 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
+            }
+            e.batch_cfl_dt(cfl, dt_out);
+        });
+}
+
+// dt that puts the largest calc_cfl number of the current so0n at cfl (no
+// reference counterpart: the reference takes dt from its caller).
+int sceuler_cfl_dt(void * h, double cfl, double * dt)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            double const v = e.cfl_max();
+            if (!(v > 0.0))
+            {
+                scamd::fail(scamd::SC_EVALUE, "EulerCore: no positive wave speed");
+            }
+            *dt = cfl / v;
+        });
+}
+
+int sceuler_march_batch(void * h, double * ms_total)
+{
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst == 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
+            }
+            hipEvent_t a = nullptr;
+            hipEvent_t b = nullptr;
+            if (ms_total != nullptr)
+            {
+                SC_HIP(hipEventCreate(&a));
+                SC_HIP(hipEventCreate(&b));
+                SC_HIP(hipEventRecord(a, e.stream));
+            }
+            e.march_batch();
+            if (ms_total != nullptr)
+            {
+                SC_HIP(hipEventRecord(b, e.stream));
+            }
+            e.sync();
+            if (ms_total != nullptr)
+            {
+                float ms = 0.0f;
+                SC_HIP(hipEventElapsedTime(&ms, a, b));
+                *ms_total = ms;
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        });
+}
+
 void sceuler_destroy(void * h) { delete static_cast<DEuler *>(h); }
 
 // ndim ncell ngstcell neq
@@ -2040,6 +2461,13 @@ int sceuler_set_param(void * h, int which, double v)
             case 0: e.sigma0 = v; break;
             case 1: e.taumin = v; break;
             case 2: e.tauscale = v; break;
+            case 3:
+                if (e.ninst > 0)
+                {
+                    scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver sets dt per instance");
+                }
+                e.dt = v;
+                break;
             default: scamd::fail(scamd::SC_EVALUE, "EulerCore: read-only or unknown parameter");
             }
         });
@@ -2079,6 +2507,10 @@ int sceuler_stage(void * h, int stage)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst > 0 && (stage == 3 || stage == 4 || stage == 5 || stage == 8))
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
+            }
             switch (stage)
             {
             case 0: e.prepare_ce(); break;
@@ -2106,6 +2538,10 @@ int sceuler_march(void * h, int64_t steps)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst > 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
+            }
             e.march(steps);
             e.sync();
         });
@@ -2114,12 +2550,39 @@ int sceuler_march(void * h, int64_t steps)
 // Enqueue a march without waiting for it.
 int sceuler_march_async(void * h, int64_t steps)
 {
-    return guard([&]() { static_cast<DEuler *>(h)->march(steps); });
+    return guard(
+        [&]()
+        {
+            DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst > 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
+            }
+            e.march(steps);
+        });
 }
 
 int sceuler_sync(void * h)
 {
     return guard([&]() { static_cast<DEuler *>(h)->sync(); });
+}
+
+// *done = 1 when all work enqueued on the solver's stream has finished,
+// 0 while some is running (no reference counterpart; for request loops).
+int sceuler_query(void * h, int * done)
+{
+    return guard(
+        [&]()
+        {
+            hipError_t const e = hipStreamQuery(static_cast<DEuler *>(h)->stream);
+            if (e == hipErrorNotReady)
+            {
+                *done = 0;
+                return;
+            }
+            SC_HIP(e);
+            *done = 1;
+        });
 }
 
 // March `steps` steps and return the average duration, in milliseconds, of
@@ -2132,6 +2595,10 @@ int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * m
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            if (e.ninst > 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
+            }
             int64_t const nsub = 2 * steps;
             std::vector<hipEvent_t> ev(static_cast<size_t>(2 * nsub + 2));
             for (auto & x : ev)

@@ -765,6 +765,22 @@ int sc1d_sync(void * h)
     return guard([&]() { static_cast<Core1D *>(h)->sync(); });
 }
 
+int sc1d_query(void * h, int * done)
+{
+    return guard(
+        [&]()
+        {
+            hipError_t const e = hipStreamQuery(static_cast<Core1D *>(h)->stream);
+            if (e == hipErrorNotReady)
+            {
+                *done = 0;
+                return;
+            }
+            SC_HIP(e);
+            *done = 1;
+        });
+}
+
 int sc1d_derived(void * h, int which, double * host_out)
 {
     return guard(

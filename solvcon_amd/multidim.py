@@ -35,6 +35,12 @@ L.register({
                                  L.c_i64]),
     'sceuler_normal_matrix': (L.c_int, [L.c_vp, L.c_i32, L.P_dbl]),
     'sceuler_reduce': (L.c_int, [L.c_vp, L.P_dbl]),
+    'sceuler_create_batch': (L.c_int, [L.c_vp, L.c_i64, L.P_i32, L.P_i64,
+                                       L.c_int, L.P_vp]),
+    'sceuler_batch_set_dt': (L.c_int, [L.c_vp, L.P_dbl]),
+    'sceuler_batch_cfl_dt': (L.c_int, [L.c_vp, L.c_dbl, L.P_dbl]),
+    'sceuler_march_batch': (L.c_int, [L.c_vp, L.P_dbl]),
+    'sceuler_cfl_dt': (L.c_int, [L.c_vp, L.c_dbl, L.P_dbl]),
     'scge_build': (L.c_int, [L.c_vp, L.P_dbl, L.c_i64, L.c_i64, L.c_i64,
                              L.c_i32, L.c_dbl, L.P_i32, L.P_dbl]),
     'scge_solve': (L.c_int, [L.c_vp, L.c_int, L.P_dbl, L.P_dbl, L.P_dbl]),
@@ -60,12 +66,25 @@ class EulerCore(object):
 
     SUBSTEP_RUN = 2
 
-    def __init__(self, mesh, time_increment, sfc=True):
+    def __init__(self, mesh, time_increment, sfc=True, batch=None):
         L.require_gpu()
         mesh._before()
         h = ctypes.c_void_p()
-        L.check(L.lib().sceuler_create(mesh._h, float(time_increment),
-                                       int(bool(sfc)), ctypes.byref(h)))
+        if batch is None:
+            self._ninst = 0
+            L.check(L.lib().sceuler_create(mesh._h, float(time_increment),
+                                           int(bool(sfc)), ctypes.byref(h)))
+        else:
+            inst, steps = batch
+            inst = np.ascontiguousarray(inst, dtype=np.int32)
+            steps = np.ascontiguousarray(steps, dtype=np.int64)
+            if inst.size != mesh.ncell:
+                raise ValueError('EulerCore batch: one instance id per cell')
+            self._ninst = steps.size
+            L.check(L.lib().sceuler_create_batch(
+                mesh._h, steps.size, L.iptr(inst),
+                steps.ctypes.data_as(L.P_i64), int(bool(sfc)),
+                ctypes.byref(h)))
         self._h = h
         self.mesh = mesh
         out = (ctypes.c_int64 * 4)()
@@ -230,6 +249,59 @@ class EulerCore(object):
         finally:
             self._after()
         return sub.value, tot.value
+
+    # -- time step (no reference counterpart) ---------------------------------------
+
+    def cfl_dt(self, cfl=0.8):
+        """dt that puts the largest calc_cfl number of the current so0n at
+        ``cfl``: cfl / max(wspd / (2 dist)), reduced on the device."""
+        self._before()
+        out = ctypes.c_double()
+        try:
+            L.check(L.lib().sceuler_cfl_dt(self._h, float(cfl),
+                                           ctypes.byref(out)))
+        finally:
+            self._after()
+        return out.value
+
+    def set_time_increment(self, dt):
+        """Change dt after construction (the reference fixes it in the
+        constructor)."""
+        L.check(L.lib().sceuler_set_param(self._h, 3, float(dt)))
+
+    # -- batched solves (no reference counterpart) ---------------------------------
+
+    def batch_set_dt(self, dt):
+        """Time increment of every batched instance."""
+        if self._ninst == 0:
+            raise RuntimeError('EulerCore: not a batched solver')
+        d = np.ascontiguousarray(dt, dtype=np.float64)
+        if d.size != self._ninst:
+            raise ValueError('EulerCore batch: one dt per instance')
+        L.check(L.lib().sceuler_batch_set_dt(self._h, L.dptr(d)))
+
+    def batch_cfl_dt(self, cfl=0.8):
+        """Set and return the dt of every instance that puts its largest
+        calc_cfl number of the current so0n at ``cfl``."""
+        self._before()
+        out = np.zeros(self._ninst)
+        try:
+            L.check(L.lib().sceuler_batch_cfl_dt(self._h, float(cfl),
+                                                 L.dptr(out)))
+        finally:
+            self._after()
+        return out
+
+    def march_batch(self):
+        """March every instance its own step count; returns the device
+        time of the march in milliseconds."""
+        self._before()
+        ms = ctypes.c_double()
+        try:
+            L.check(L.lib().sceuler_march_batch(self._h, ctypes.byref(ms)))
+        finally:
+            self._after()
+        return ms.value
 
     # -- boundary conditions ------------------------------------------------------
 

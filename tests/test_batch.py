@@ -1,0 +1,153 @@
+"""Batched form: many independent solves in one device march.
+
+CPU: the disjoint union of mesh definitions keeps every instance's cells
+and nodes.  GPU: every instance marched in the batch is bit-identical, on
+every body and ghost row of every table, to the same instance marched alone
+by the plain EulerCore with the same dt.  That holds for mixed sizes, step
+counts, cell types and dimensions.
+"""
+
+import numpy as np
+import pytest
+
+import meshdefs as MD
+
+
+def _defs2d():
+    from solvcon_amd.gen import tri_mesh_def
+    d = [tri_mesh_def(m, seed=10 + m) for m in (3, 5, 8, 4)]
+    d.append(MD.quad_channel(6, 3, 2.0, 1.0))
+    d.append(tri_mesh_def(2, seed=7))
+    return d
+
+
+def test_union_def_keeps_instances():
+    from solvcon_amd.batch import union_def
+    defs = _defs2d()
+    (ndim, ndcrd, cltpn, clnds), inst, coff, noff = union_def(defs)
+    assert ndim == 2 and inst.size == coff[-1] == cltpn.size
+    for k, d in enumerate(defs):
+        c = np.asarray(d[3])
+        u = clnds[coff[k]:coff[k + 1]]
+        assert (inst[coff[k]:coff[k + 1]] == k).all()
+        assert (u[:, 0] == c[:, 0]).all()
+        for r in range(c.shape[0]):
+            n = c[r, 0]
+            got = ndcrd[u[r, 1:1 + n]]
+            want = np.asarray(d[1], dtype=np.float64)[c[r, 1:1 + n]]
+            assert (got == want).all()
+
+
+# ----------------------------------------------------------------- GPU -----
+
+TABLES = ('so0n', 'so1n', 'so0c', 'so1c', 'so0t', 'cflo', 'cflc')
+
+
+def _alone(defn, state, dt, steps, gamma=1.4):
+    from solvcon_amd.batch import build_mesh
+    from solvcon_amd import EulerCore
+    mh = build_mesh(defn)
+    ec = EulerCore(mh, dt)
+    ec.init_solution(gamma, 1.0, [0.0] * defn[0], 1.0)
+    ec.so0n.ndarray[mh.ngstcell:] = state
+    ec.add_nonrefl(mh.bndfcs.ndarray[:, 0])
+    ec.bc_soln()
+    ec.bc_dsoln()
+    ec.march(steps)
+    return mh, ec
+
+
+def _check_batch(defs, states, steps):
+    from solvcon_amd.batch import BatchedEuler
+    b = BatchedEuler(defs, states, steps)
+    b.march()
+    gofs = 0
+    for k, d in enumerate(defs):
+        print('  instance', k, 'cells', len(d[2]), 'steps', steps[k],
+              flush=True)
+        mh, ec = _alone(d, states[k], b.dts[k], steps[k])
+        ng = mh.ngstcell
+        for name in TABLES:
+            a = getattr(ec, name).ndarray
+            u = getattr(b.core, name).ndarray
+            body = u[b.ngst + b.coff[k]:b.ngst + b.coff[k + 1]]
+            assert np.array_equal(a[ng:], body), (k, name)
+            # Ghost rows of instance k: a block of the union's ghost rows in
+            # reverse order (ghost -1 is the last ghost row).
+            ug = u[:b.ngst][::-1][gofs:gofs + ng]
+            assert np.array_equal(a[:ng][::-1], ug), (k, name, 'ghost')
+        gofs += ng
+    assert gofs == b.ngst
+
+
+@pytest.mark.gpu
+def test_gpu_batch_equals_alone_2d():
+    print('case: batch of 6 2D instances', flush=True)
+    from solvcon_amd.gen import riemann_box, quadrant_state
+    from solvcon_amd.batch import build_mesh
+    defs = _defs2d()
+    states = []
+    for k, d in enumerate(defs):
+        mh = build_mesh(d)
+        cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
+        rho, p = riemann_box(30 + k)
+        states.append(quadrant_state(cen / cen.max(0), rho, p))
+    _check_batch(defs, states, [4, 7, 2, 7, 5, 1])
+
+
+@pytest.mark.gpu
+def test_gpu_batch_equals_alone_3d():
+    print('case: batch of 3 3D mixed instances', flush=True)
+    from solvcon_amd.gen import mixed_mesh_def, blast_state
+    from solvcon_amd.batch import build_mesh
+    defs = [mixed_mesh_def(3, seed=3), mixed_mesh_def(4, seed=5),
+            MD.hex_block(3, 2, 2, jitter=0.1, seed=2)]
+    states = []
+    for d in defs:
+        mh = build_mesh(d)
+        cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
+        states.append(blast_state(cen, r=0.4, center=cen.mean(0)))
+    _check_batch(defs, states, [2, 3, 1])
+
+
+@pytest.mark.gpu
+def test_gpu_batch_config_sample():
+    # A few instances of the batched workload's own generator.
+    print('case: batch config sample', flush=True)
+    from solvcon_amd.batch import (batch_config, config_defs, config_states,
+                                   BatchedEuler)
+    m, ms, rs, steps = batch_config(ninst=5, lo=50, hi=800, smin=2, smax=9)
+    defs = config_defs(m, ms)
+    st = config_states(rs)
+    b = BatchedEuler(defs, st, steps)
+    b.march()
+    for k in range(len(defs)):
+        from solvcon_amd.batch import build_mesh
+        mh = build_mesh(defs[k])
+        cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
+        _, ec = _alone(defs[k], st(cen, k), b.dts[k], steps[k])
+        assert np.array_equal(ec.so0n.ndarray[ec.ngstcell:],
+                              b.body('so0n', k))
+
+
+@pytest.mark.gpu
+def test_gpu_batch_guards():
+    print('case: batch API guards', flush=True)
+    from solvcon_amd.batch import BatchedEuler
+    from solvcon_amd.gen import tri_mesh_def
+    from solvcon_amd import EulerCore
+    d = tri_mesh_def(3, seed=1)
+    st = np.zeros((18, 4))
+    st[:, 0] = 1.0
+    st[:, 3] = 2.5
+    b = BatchedEuler([d, d], [st, st], [2, 3], dts=[0.01, 0.02])
+    with pytest.raises(RuntimeError):
+        b.core.march(1)
+    with pytest.raises(RuntimeError):
+        b.core.calc_soln()
+    b.march()
+    ec = EulerCore(b.mesh, 0.01)
+    with pytest.raises(RuntimeError):
+        ec.batch_set_dt([0.1])
+    with pytest.raises(ValueError):
+        BatchedEuler([d], [st], [0])
