@@ -133,7 +133,9 @@ void sceuler_destroy(void * h);
 int sceuler_counts(void * h, int64_t * out);
 /* sigma0 (0), taumin (1), tauscale (2), time_increment (3),
  * euler.hpp:95-102; the reference fixes time_increment at construction,
- * setting it here (not on a batched solver) has no counterpart. */
+ * setting it here (not on a batched solver) has no counterpart.  Read-only
+ * 4 (no counterpart): the autotuned kernel form, bit t set when the cells
+ * of type t run the split kernel pair, -1 before the first substep. */
 double sceuler_get_param(void * h, int which);
 int sceuler_set_param(void * h, int which, double v);
 /* info[4]: rows, elements per row, ghost rows, rank of the table. */

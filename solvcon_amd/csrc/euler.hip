@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <cmath>
 #include <memory>
@@ -523,9 +524,13 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
                 udf[f][k] = val;
             }
         }
-        // Displacement-matrix adjugates and determinants of every FGE;
-        // they depend only on geometry, so each is formed once.
-        double adj[NFGE][ND][ND];
+        // Displacement-matrix adjugates and determinants of every FGE
+        // depend only on geometry.  In 2D all adjugates stay in registers
+        // across the equations; in 3D (8 x 9 values for a hexahedron) each
+        // is formed again per equation, which gives the same values and
+        // keeps the register footprint down.
+        constexpr bool ADJ_ONCE = NFGE * ND * ND <= 16;
+        double adj[ADJ_ONCE ? NFGE : 1][ND][ND];
         double det[NFGE];
 #pragma unroll
         for (int e = 0; e < NFGE; ++e)
@@ -540,7 +545,10 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
                     dst[v][d] = idis[TR::fge(e, v) - 1][d];
                 }
             }
-            ge_adj<ND>(dst, adj[e]);
+            if constexpr (ADJ_ONCE)
+            {
+                ge_adj<ND>(dst, adj[e]);
+            }
             det[e] = ge_det<ND>(dst);
         }
         constexpr double ofg1 = 1.0 / NFGE;
@@ -553,6 +561,33 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int e = 0; e < NFGE; ++e)
             {
+                double ae[ND][ND];
+                if constexpr (ADJ_ONCE)
+                {
+#pragma unroll
+                    for (int a = 0; a < ND; ++a)
+                    {
+#pragma unroll
+                        for (int b = 0; b < ND; ++b)
+                        {
+                            ae[a][b] = adj[e][a][b];
+                        }
+                    }
+                }
+                else
+                {
+                    double dst[ND][ND];
+#pragma unroll
+                    for (int v = 0; v < ND; ++v)
+                    {
+#pragma unroll
+                        for (int d = 0; d < ND; ++d)
+                        {
+                            dst[v][d] = idis[TR::fge(e, v) - 1][d];
+                        }
+                    }
+                    ge_adj<ND>(dst, ae);
+                }
                 double r[ND];
 #pragma unroll
                 for (int a = 0; a < ND; ++a)
@@ -561,7 +596,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                     for (int b = 0; b < ND; ++b)
                     {
-                        r[a] += adj[e][a][b] * udf[TR::fge(e, b) - 1][k];
+                        r[a] += ae[a][b] * udf[TR::fge(e, b) - 1][k];
                     }
                 }
                 double sq = 0.0;
@@ -1404,35 +1439,58 @@ __global__ void k_batch_cflmax(int64_t R, int64_t G, int64_t off, int64_t n, dou
     constexpr int ND = Traits<T>::ND;
     constexpr int NEQ = ND + 2;
     int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
+    int64_t const i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u);
+    if (i0 >= n)
     {
-        return;
+        return; // the whole wavefront is past the end
     }
-    int64_t const row = G + off + i;
-    double u0[NEQ];
+    bool const valid = i < n;
+    unsigned long long bits = 0;
+    int32_t slot = 0;
+    if (valid)
+    {
+        int64_t const row = G + off + i;
+        double u0[NEQ];
 #pragma unroll
-    for (int k = 0; k < NEQ; ++k)
-    {
-        u0[k] = s0n[k * R + row];
-    }
-    double const dist = geo[LY::FD * n + i];
-    double const ga = gam[row];
-    double const density = u0[0];
-    double momsq = 0.0;
+        for (int k = 0; k < NEQ; ++k)
+        {
+            u0[k] = s0n[k * R + row];
+        }
+        double const dist = geo[LY::FD * n + i];
+        double const ga = gam[row];
+        double const density = u0[0];
+        double momsq = 0.0;
 #pragma unroll
-    for (int d = 0; d < ND; ++d)
-    {
-        momsq += u0[1 + d] * u0[1 + d];
+        for (int d = 0; d < ND; ++d)
+        {
+            momsq += u0[1 + d] * u0[1 + d];
+        }
+        double const ke = momsq / (2.0 * density);
+        double const pr = (ga - 1.0) * (u0[NEQ - 1] - ke);
+        double const pra = (pr + fabs(pr)) / 2.0;
+        double const wspd = sqrt(ga * pra / density) + sqrt(momsq) / density;
+        double const v = 0.5 * wspd / dist;
+        bits = v >= 0.0 ? static_cast<unsigned long long>(__double_as_longlong(v)) : 0ull;
+        slot = inst != nullptr ? inst[off + i] : 0;
     }
-    double const ke = momsq / (2.0 * density);
-    double const pr = (ga - 1.0) * (u0[NEQ - 1] - ke);
-    double const pra = (pr + fabs(pr)) / 2.0;
-    double const wspd = sqrt(ga * pra / density) + sqrt(momsq) / density;
-    double const v = 0.5 * wspd / dist;
-    if (v >= 0.0)
+    // One atomic per wavefront when all its cells belong to one instance
+    // (always for a single mesh): a maximum over the lanes first.
+    int32_t const s0 = __shfl(slot, 0);
+    if (__all(!valid || slot == s0))
     {
-        atomicMax(&cmax[inst != nullptr ? inst[off + i] : 0],
-                  static_cast<unsigned long long>(__double_as_longlong(v)));
+        for (int m = 32; m > 0; m >>= 1)
+        {
+            unsigned long long const o = __shfl_xor(bits, m);
+            bits = o > bits ? o : bits;
+        }
+        if ((threadIdx.x & 63u) == 0)
+        {
+            atomicMax(&cmax[s0], bits);
+        }
+    }
+    else if (valid)
+    {
+        atomicMax(&cmax[slot], bits);
     }
 }
 
@@ -1458,6 +1516,8 @@ struct DEuler
     double taumin = 0.0;
     double tauscale = 1.0;
     bool use_sfc = true;
+    int tuned = 0;              // 0 until the first substep has timed both forms
+    std::vector<char> gsplit;   // per group: 1 runs the split pair
     hipStream_t stream = nullptr;
 
     struct Group
@@ -1531,29 +1591,36 @@ struct DEuler
         return p;
     }
 
+    // One kernel of mode MODE over the first cnt cells of a type group.
+    template <int MODE>
+    void run_group(KP const & p, Group const & gr, int64_t cnt)
+    {
+        if (cnt <= 0)
+        {
+            return;
+        }
+        GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, cnt};
+        unsigned const nb = grid_for(cnt, BLK);
+        switch (gr.type)
+        {
+        case CT_TRI: k_cell<CT_TRI, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        case CT_QUAD: k_cell<CT_QUAD, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        case CT_TET: k_cell<CT_TET, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        case CT_HEX: k_cell<CT_HEX, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        case CT_PRISM: k_cell<CT_PRISM, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        case CT_PYRAMID: k_cell<CT_PYRAMID, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        default: fail(SC_ERUNTIME, "EulerCore: unsupported cell type in group");
+        }
+        SC_HIP(hipGetLastError());
+    }
+
     template <int MODE>
     void run_cells()
     {
         KP const p = params();
-        for (auto & gr : groups)
+        for (auto const & gr : groups)
         {
-            if (gr.n == 0)
-            {
-                continue;
-            }
-            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, gr.n};
-            unsigned const nb = grid_for(gr.n, BLK);
-            switch (gr.type)
-            {
-            case CT_TRI: k_cell<CT_TRI, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_QUAD: k_cell<CT_QUAD, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_TET: k_cell<CT_TET, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_HEX: k_cell<CT_HEX, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_PRISM: k_cell<CT_PRISM, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_PYRAMID: k_cell<CT_PYRAMID, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-            default: fail(SC_ERUNTIME, "EulerCore: unsupported cell type in group");
-            }
-            SC_HIP(hipGetLastError());
+            run_group<MODE>(p, gr, gr.n);
         }
     }
 
@@ -1673,10 +1740,129 @@ struct DEuler
 
     void update() { cur = 1 - cur; }
 
+    // One substep.  Every type group runs either the fused kernel or the
+    // split pair that reads so0t from memory: calc_solt over every row
+    // (once per substep when any group is split), then soln + BC + CFL,
+    // then dsoln.  Both forms compute the same values; the split pair
+    // needs fewer registers per kernel and wins where the fused kernel
+    // runs at one wavefront per SIMD.
+    static constexpr int M_SPLIT1 = M_SOLN | M_BC | M_CFL | M_STMEM;
+    static constexpr int M_SPLIT2 = M_DSOLN | M_STMEM;
+
     void substep_fused()
     {
+        if (tuned == 0)
+        {
+            tune_substep();
+            return;
+        }
         update();
-        run_cells<M_FUSED>();
+        KP const p = params();
+        bool any = false;
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+        {
+            any = any || gsplit[gi] != 0;
+        }
+        if (any)
+        {
+            calc_solt();
+        }
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+        {
+            Group const & gr = groups[gi];
+            if (gsplit[gi] != 0)
+            {
+                run_group<M_SPLIT1>(p, gr, gr.n);
+                run_group<M_SPLIT2>(p, gr, gr.n);
+            }
+            else
+            {
+                run_group<M_FUSED>(p, gr, gr.n);
+            }
+        }
+    }
+
+    // The autotuner: the first substep of a march runs both forms of
+    // every group (the split pair rewrites the same outputs with the same
+    // values), times them with HIP events, and keeps for each group the
+    // faster form, counting calc_solt once if any group is split.
+    void tune_substep()
+    {
+        update();
+        KP const p = params();
+        size_t const ng = groups.size();
+        std::vector<hipEvent_t> ev(2 + 4 * ng);
+        for (auto & x : ev)
+        {
+            SC_HIP(hipEventCreate(&x));
+        }
+        SC_HIP(hipEventRecord(ev[0], stream));
+        calc_solt();
+        SC_HIP(hipEventRecord(ev[1], stream));
+        for (size_t gi = 0; gi < ng; ++gi)
+        {
+            Group const & gr = groups[gi];
+            SC_HIP(hipEventRecord(ev[2 + 4 * gi], stream));
+            run_group<M_FUSED>(p, gr, gr.n);
+            SC_HIP(hipEventRecord(ev[3 + 4 * gi], stream));
+            run_group<M_SPLIT1>(p, gr, gr.n);
+            run_group<M_SPLIT2>(p, gr, gr.n);
+            SC_HIP(hipEventRecord(ev[4 + 4 * gi], stream));
+            SC_HIP(hipEventRecord(ev[5 + 4 * gi], stream));
+        }
+        SC_HIP(hipEventSynchronize(ev[ng > 0 ? 5 + 4 * (ng - 1) : 1]));
+        float solt = 0.0f;
+        SC_HIP(hipEventElapsedTime(&solt, ev[0], ev[1]));
+        std::vector<float> tf(ng), ts(ng);
+        for (size_t gi = 0; gi < ng; ++gi)
+        {
+            SC_HIP(hipEventElapsedTime(&tf[gi], ev[2 + 4 * gi], ev[3 + 4 * gi]));
+            SC_HIP(hipEventElapsedTime(&ts[gi], ev[3 + 4 * gi], ev[4 + 4 * gi]));
+        }
+        for (auto & x : ev)
+        {
+            (void)hipEventDestroy(x);
+        }
+        // Best subset of split groups (at most six groups).
+        double best = -1.0;
+        unsigned bestmask = 0;
+        for (unsigned mask = 0; mask < (1u << ng); ++mask)
+        {
+            double t = mask != 0 ? solt : 0.0;
+            for (size_t gi = 0; gi < ng; ++gi)
+            {
+                t += ((mask >> gi) & 1u) != 0 ? ts[gi] : tf[gi];
+            }
+            if (best < 0.0 || t < best)
+            {
+                best = t;
+                bestmask = mask;
+            }
+        }
+        for (size_t gi = 0; gi < ng; ++gi)
+        {
+            gsplit[gi] = static_cast<char>((bestmask >> gi) & 1u);
+        }
+        tuned = 1;
+    }
+
+    // Bit t set when the group of cell type t runs the split pair; -1
+    // before the first substep has tuned the forms.
+    int64_t split_mask() const
+    {
+        if (tuned == 0)
+        {
+            return -1;
+        }
+        int64_t m = 0;
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+        {
+            if (gsplit[gi] != 0)
+            {
+                m |= int64_t(1) << groups[gi].type;
+            }
+        }
+        return m;
     }
 
     void march(int64_t steps)
@@ -1857,6 +2043,25 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         bqdt.alloc(ninst);
         std::vector<double> d(static_cast<size_t>(ninst), time_increment);
         batch_set_dt(d.data());
+    }
+    // Fused or split per group: timed by the first substep unless
+    // SC_EULER_SPLIT fixes it (0 all fused, 1 all split, a: split in 3D).
+    gsplit.assign(groups.size(), 0);
+    if (char const * env = std::getenv("SC_EULER_SPLIT"))
+    {
+        char const c = env[0];
+        if (c == '0' || c == '1' || c == 'a')
+        {
+            for (auto & x : gsplit)
+            {
+                x = static_cast<char>(c == '1' || (c == 'a' && ndim == 3));
+            }
+            tuned = 1;
+        }
+    }
+    if (ninst > 0)
+    {
+        tuned = 1; // the batched march runs the fused kernel
     }
     for (auto & gr : groups)
     {
@@ -2446,6 +2651,7 @@ double sceuler_get_param(void * h, int which)
     case 0: return e.sigma0;
     case 1: return e.taumin;
     case 2: return e.tauscale;
+    case 4: return static_cast<double>(e.split_mask());
     default: return e.dt;
     }
 }
@@ -2606,12 +2812,20 @@ int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * m
                 SC_HIP(hipEventCreate(&x));
             }
             SC_HIP(hipEventRecord(ev[2 * nsub], e.stream));
+            int64_t counted = 0;
             for (int64_t it = 0; it < nsub; ++it)
             {
-                e.update();
+                if (e.tuned == 0)
+                {
+                    e.substep_fused(); // the tuning substep, not counted
+                    SC_HIP(hipEventRecord(ev[2 * it], e.stream));
+                    SC_HIP(hipEventRecord(ev[2 * it + 1], e.stream));
+                    continue;
+                }
                 SC_HIP(hipEventRecord(ev[2 * it], e.stream));
-                e.run_cells<scamd::M_FUSED>();
+                e.substep_fused();
                 SC_HIP(hipEventRecord(ev[2 * it + 1], e.stream));
+                ++counted;
             }
             if (steps > 0)
             {
@@ -2628,7 +2842,7 @@ int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * m
             }
             float tot = 0.0f;
             SC_HIP(hipEventElapsedTime(&tot, ev[2 * nsub], ev[2 * nsub + 1]));
-            *ms_substep = nsub > 0 ? sub / static_cast<double>(nsub) : 0.0;
+            *ms_substep = counted > 0 ? sub / static_cast<double>(counted) : 0.0;
             *ms_total = tot;
             for (auto & x : ev)
             {

@@ -174,6 +174,17 @@ class EulerCore(object):
             L.check(L.lib().sceuler_set_param(self._h, i, float(v)))
         return property(get, put)
 
+    @property
+    def kernel_forms(self):
+        """{cell type id: 'fused' | 'split'} chosen by the autotuner of
+        the first substep (None before it; no reference counterpart)."""
+        m = int(L.lib().sceuler_get_param(self._h, 4))
+        if m < 0:
+            return None
+        types = np.unique(self.mesh.cltpn.ndarray[self.mesh.ngstcell:])
+        return {int(t): 'split' if (m >> int(t)) & 1 else 'fused'
+                for t in types}
+
     sigma0 = _param(0)
     taumin = _param(1)
     tauscale = _param(2)
