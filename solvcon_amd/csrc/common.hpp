@@ -78,12 +78,37 @@ int guard(F && f)
     }
 }
 
-// Device buffer owned by a handle.
+// The stream that device allocations of the calling thread are ordered on:
+// a handle sets it (AllocScope) while it works, so its buffers come from
+// the stream-ordered pool (hipMallocAsync / hipFreeAsync) and freeing one
+// never waits for the work of other handles' streams, as hipFree would.
+inline hipStream_t & alloc_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+
+struct AllocScope
+{
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s)
+        : prev(alloc_stream())
+    {
+        alloc_stream() = s;
+    }
+    ~AllocScope() { alloc_stream() = prev; }
+    AllocScope(AllocScope const &) = delete;
+    AllocScope & operator=(AllocScope const &) = delete;
+};
+
+// Device buffer owned by a handle; freed on the stream it was allocated on
+// (or with hipFree when it was allocated outside any AllocScope).
 template <typename T>
 struct DevBuf
 {
     T * ptr = nullptr;
     int64_t count = 0;
+    hipStream_t stream = nullptr;
 
     DevBuf() = default;
     DevBuf(DevBuf const &) = delete;
@@ -91,9 +116,11 @@ struct DevBuf
     DevBuf(DevBuf && o) noexcept
         : ptr(o.ptr)
         , count(o.count)
+        , stream(o.stream)
     {
         o.ptr = nullptr;
         o.count = 0;
+        o.stream = nullptr;
     }
     DevBuf & operator=(DevBuf && o) noexcept
     {
@@ -102,8 +129,10 @@ struct DevBuf
             release();
             ptr = o.ptr;
             count = o.count;
+            stream = o.stream;
             o.ptr = nullptr;
             o.count = 0;
+            o.stream = nullptr;
         }
         return *this;
     }
@@ -114,7 +143,16 @@ struct DevBuf
         release();
         if (n > 0)
         {
-            SC_HIP(hipMalloc(&ptr, sizeof(T) * static_cast<size_t>(n)));
+            stream = alloc_stream();
+            size_t const nb = sizeof(T) * static_cast<size_t>(n);
+            if (stream != nullptr)
+            {
+                SC_HIP(hipMallocAsync(reinterpret_cast<void **>(&ptr), nb, stream));
+            }
+            else
+            {
+                SC_HIP(hipMalloc(&ptr, nb));
+            }
         }
         count = n;
     }
@@ -122,12 +160,39 @@ struct DevBuf
     {
         if (ptr != nullptr)
         {
-            (void)hipFree(ptr);
+            if (stream != nullptr)
+            {
+                (void)hipFreeAsync(ptr, stream);
+            }
+            else
+            {
+                (void)hipFree(ptr);
+            }
             ptr = nullptr;
         }
         count = 0;
+        stream = nullptr;
     }
     size_t bytes() const { return sizeof(T) * static_cast<size_t>(count); }
+};
+
+// A non-blocking stream at the default priority, synchronised and destroyed
+// last: a handle declares it before its buffers, so they are freed on it
+// while it still exists.
+struct StreamHolder
+{
+    hipStream_t s = nullptr;
+    StreamHolder() { SC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~StreamHolder()
+    {
+        if (s != nullptr)
+        {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+    StreamHolder(StreamHolder const &) = delete;
+    StreamHolder & operator=(StreamHolder const &) = delete;
 };
 
 inline unsigned grid_for(int64_t n, unsigned block)

@@ -7,9 +7,11 @@
  * Layout in HBM.  Body cells are grouped by cell type and, inside a group,
  * ordered along a Morton curve of their centroids; ghost rows keep the
  * reference order and come first.  A row is therefore 0..G-1 for ghosts and
- * G + (internal position) for body cells.  Solution tables are
- * structure-of-arrays, table[field * R + row] with R = G + N, so each
- * field of a wavefront's cells is one contiguous load.  Each type group
+ * G + (internal position) for body cells.  Solution tables are row-major
+ * records as in the reference, table[row * width + component] (so0 and so0t
+ * with neq components, so1 with neq * ndim, the solution points with
+ * ndim), so gathering a neighbour's state touches one or two cache lines
+ * instead of one line per component.  Each type group
  * stores its CE geometry as geo[field * n + cell] (CE volumes, BCE
  * centroids, sub-face centroids and normals, CFL distance) and its
  * neighbour rows as nbr[face * n + cell].
@@ -82,6 +84,7 @@ enum Mode : int
     M_DSOLN = 8,
     M_STMEM = 16,
     M_BATCH = 32,
+    M_W3 = 64, // compile for three wavefronts per SIMD (registers capped at 168)
     M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
 };
 
@@ -148,7 +151,7 @@ void fill(T * a, int64_t n, T v, hipStream_t s)
 // --------------------------------------------------------------------------
 
 template <int T, int MODE>
-__global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV const g)
+__global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k_cell(KP const p, GV const g)
 {
     using TR = Traits<T>;
     using LY = Layout<T>;
@@ -163,7 +166,6 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
         return;
     }
     int64_t const n = g.n;
-    int64_t const R = p.R;
     int64_t const row = p.G + g.off + i;
     double const * geo = g.geo + i;
     double hdt = p.hdt;
@@ -189,7 +191,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
         for (int d = 0; d < ND; ++d)
         {
-            jce[f][d] = p.cen[d * R + nb[f]];
+            jce[f][d] = p.cen[nb[f] * ND + d];
         }
     }
     double st[NF][NEQ];
@@ -213,11 +215,11 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                js[k] = p.s0c[k * R + j];
+                js[k] = p.s0c[(j) * NEQ + k];
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
-                    j1[k][d] = p.s1c[(k * ND + d) * R + j];
+                    j1[k][d] = p.s1c[(j) * (NEQ * ND) + k * ND + d];
                 }
             }
             double const bvol = geo[(LY::FV + 1 + f) * n];
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    st[f][k] = p.st[k * R + j];
+                    st[f][k] = p.st[(j) * NEQ + k];
                 }
             }
             else
@@ -316,7 +318,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                     for (int k = 0; k < NEQ; ++k)
                     {
-                        p.s0n[k * R + j] = o0[k];
+                        p.s0n[(j) * NEQ + k] = o0[k];
                     }
                     double in1[NEQ][ND];
 #pragma unroll
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                         for (int d = 0; d < ND; ++d)
                         {
-                            in1[k][d] = p.s1c[(k * ND + d) * R + row];
+                            in1[k][d] = p.s1c[(row) * (NEQ * ND) + k * ND + d];
                         }
                     }
                     double o1[NEQ][ND];
@@ -336,7 +338,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                         for (int d = 0; d < ND; ++d)
                         {
-                            p.s1n[(k * ND + d) * R + j] = o1[k][d];
+                            p.s1n[(j) * (NEQ * ND) + k * ND + d] = o1[k][d];
                         }
                     }
                 }
@@ -353,7 +355,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                u0[k] = p.s0n[k * R + row];
+                u0[k] = p.s0n[(row) * NEQ + k];
             }
         }
         double const dist = geo[LY::FD * n];
@@ -383,7 +385,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
         {
-            p.s0n[k * R + row] = u0[k];
+            p.s0n[(row) * NEQ + k] = u0[k];
         }
     }
 
@@ -397,7 +399,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                u0[k] = p.s0n[k * R + row];
+                u0[k] = p.s0n[(row) * NEQ + k];
             }
         }
         if constexpr ((MODE & M_SOLN) == 0)
@@ -408,7 +410,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    st[f][k] = p.st[k * R + nb[f]];
+                    st[f][k] = p.st[nb[f] * NEQ + k];
                 }
             }
         }
@@ -419,7 +421,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
         for (int d = 0; d < ND; ++d)
         {
-            ice[d] = p.cen[d * R + row];
+            ice[d] = p.cen[(row) * ND + d];
         }
         double gp[NF][ND];
 #pragma unroll
@@ -515,11 +517,11 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                double val = p.s0c[k * R + j] + hdt * st[f][k] - u0[k];
+                double val = p.s0c[(j) * NEQ + k] + hdt * st[f][k] - u0[k];
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
-                    val += jdis[d] * p.s1c[(k * ND + d) * R + j];
+                    val += jdis[d] * p.s1c[(j) * (NEQ * ND) + k * ND + d];
                 }
                 udf[f][k] = val;
             }
@@ -641,7 +643,7 @@ __global__ void __launch_bounds__(BLK, SC_CELL_WAVES) k_cell(KP const p, GV cons
 #pragma unroll
             for (int d = 0; d < ND; ++d)
             {
-                p.s1n[(k * ND + d) * R + row] = out[d];
+                p.s1n[(row) * (NEQ * ND) + k * ND + d] = out[d];
             }
         }
     }
@@ -657,17 +659,16 @@ __global__ void k_solt(KP const p)
     {
         return;
     }
-    int64_t const R = p.R;
     double s0[NEQ];
     double s1[NEQ][ND];
 #pragma unroll
     for (int k = 0; k < NEQ; ++k)
     {
-        s0[k] = p.s0c[k * R + r];
+        s0[k] = p.s0c[(r) * NEQ + k];
 #pragma unroll
         for (int d = 0; d < ND; ++d)
         {
-            s1[k][d] = p.s1c[(k * ND + d) * R + r];
+            s1[k][d] = p.s1c[(r) * (NEQ * ND) + k * ND + d];
         }
     }
     typename JacOf<ND>::type jac;
@@ -677,7 +678,7 @@ __global__ void k_solt(KP const p)
 #pragma unroll
     for (int k = 0; k < NEQ; ++k)
     {
-        p.st[k * R + r] = st[k];
+        p.st[(r) * NEQ + k] = st[k];
     }
 }
 
@@ -696,7 +697,6 @@ __global__ void k_bc(KP const p, int32_t const * gsrc)
     {
         return;
     }
-    int64_t const R = p.R;
     int64_t const src = gsrc[j];
     double mat[ND][ND];
 #pragma unroll
@@ -715,13 +715,13 @@ __global__ void k_bc(KP const p, int32_t const * gsrc)
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
         {
-            in[k] = p.s0n[k * R + src];
+            in[k] = p.s0n[(src) * NEQ + k];
         }
         bc_order0<ND>(kind, mat, p.inlet + static_cast<int64_t>(p.ginl[j]) * (ND + 3), in, out);
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
         {
-            p.s0n[k * R + j] = out[k];
+            p.s0n[(j) * NEQ + k] = out[k];
         }
     }
     else
@@ -734,7 +734,7 @@ __global__ void k_bc(KP const p, int32_t const * gsrc)
 #pragma unroll
             for (int d = 0; d < ND; ++d)
             {
-                in[k][d] = p.s1c[(k * ND + d) * R + src];
+                in[k][d] = p.s1c[(src) * (NEQ * ND) + k * ND + d];
             }
         }
         bc_order1<ND>(kind, mat, in, out);
@@ -744,7 +744,7 @@ __global__ void k_bc(KP const p, int32_t const * gsrc)
 #pragma unroll
             for (int d = 0; d < ND; ++d)
             {
-                p.s1n[(k * ND + d) * R + j] = out[k][d];
+                p.s1n[(j) * (NEQ * ND) + k * ND + d] = out[k][d];
             }
         }
     }
@@ -932,7 +932,7 @@ __global__ void k_prepare(MeshView const m, int64_t off, int64_t n, int32_t * nb
 #pragma unroll
     for (int d = 0; d < ND; ++d)
     {
-        cen[d * R + row] = cc[d] / volc;
+        cen[(row) * ND + d] = cc[d] / volc;
     }
 }
 
@@ -958,7 +958,7 @@ __global__ void k_dist(int64_t G, int64_t off, int64_t n, double * geo, double c
 #pragma unroll
         for (int d = 0; d < ND; ++d)
         {
-            double const diff = geo[(LY::FB + f * ND + d) * n + i] - cen[d * R + row];
+            double const diff = geo[(LY::FB + f * ND + d) * n + i] - cen[(row) * ND + d];
             d2 += diff * diff;
         }
         double const s = sqrt(d2);
@@ -975,7 +975,7 @@ __global__ void k_cen_ghost(int ndim, int64_t G, double const * clcnd_full, doub
     {
         for (int d = 0; d < ndim; ++d)
         {
-            cen[d * R + r] = clcnd_full[r * ndim + d];
+            cen[r * ndim + d] = clcnd_full[r * ndim + d];
         }
     }
 }
@@ -1004,11 +1004,11 @@ __global__ void k_state(int64_t R, int64_t G, int w, int32_t const * int_of_ref,
     {
         if (DOWN)
         {
-            api[r * w + c] = soa[c * R + ir];
+            api[r * w + c] = soa[ir * w + c];
         }
         else
         {
-            soa[c * R + ir] = api[r * w + c];
+            soa[ir * w + c] = api[r * w + c];
         }
     }
 }
@@ -1050,11 +1050,11 @@ __global__ void k_geom(int which, int64_t G, int64_t off, int64_t n, int32_t con
         {
             if (DOWN)
             {
-                a[d] = cen[d * R + row];
+                a[d] = cen[(row) * ND + d];
             }
             else
             {
-                cen[d * R + row] = a[d];
+                cen[(row) * ND + d] = a[d];
             }
         }
         for (int c = 0; c < TR::NF * ND; ++c)
@@ -1109,12 +1109,12 @@ __global__ void k_init(int64_t R, int64_t G, int neq, int ndim, double ga, doubl
     if (r >= G)
     {
         double const v[3] = {v0, v1, v2};
-        s0n[r] = rho;
+        s0n[r * neq] = rho;
         for (int d = 0; d < ndim; ++d)
         {
-            s0n[(1 + d) * R + r] = rho * v[d];
+            s0n[r * neq + 1 + d] = rho * v[d];
         }
-        s0n[(neq - 1) * R + r] = energy;
+        s0n[r * neq + neq - 1] = energy;
     }
 }
 
@@ -1374,7 +1374,7 @@ __global__ void k_set_bc(int32_t n, int32_t const * faces, int32_t const * fccls
 
 // Sums over the body cells for diagnostics: density * volume and the CFL
 // range (EulerField.calc_overall_mass / calc_cfl_range).
-__global__ void k_reduce(int64_t R, int64_t G, double const * s0n, double const * clvol_b, int32_t const * ref_of_int,
+__global__ void k_reduce(int64_t R, int64_t G, int neq, double const * s0n, double const * clvol_b, int32_t const * ref_of_int,
                          double const * cflo, double * part)
 {
     __shared__ double sm[BLK];
@@ -1386,7 +1386,7 @@ __global__ void k_reduce(int64_t R, int64_t G, double const * s0n, double const 
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < R - G;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x)
     {
-        a += s0n[G + i] * clvol_b[ref_of_int[i]];
+        a += s0n[(G + i) * neq] * clvol_b[ref_of_int[i]];
         lo = fmin(lo, cflo[G + i]);
         hi = fmax(hi, cflo[G + i]);
     }
@@ -1454,7 +1454,7 @@ __global__ void k_batch_cflmax(int64_t R, int64_t G, int64_t off, int64_t n, dou
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
         {
-            u0[k] = s0n[k * R + row];
+            u0[k] = s0n[(row) * NEQ + k];
         }
         double const dist = geo[LY::FD * n + i];
         double const ga = gam[row];
@@ -1517,8 +1517,9 @@ struct DEuler
     double tauscale = 1.0;
     bool use_sfc = true;
     int tuned = 0;              // 0 until the first substep has timed both forms
-    std::vector<char> gsplit;   // per group: 1 runs the split pair
-    hipStream_t stream = nullptr;
+    std::vector<char> gsplit;   // per group: 0 fused, 1 split pair, 2 fused at 3 waves
+    StreamHolder sh;            // declared before the buffers: freed last
+    hipStream_t stream = sh.s;
 
     struct Group
     {
@@ -1550,14 +1551,7 @@ struct DEuler
 
     DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch = 0, int32_t const * inst_of_cell = nullptr,
            int64_t const * steps = nullptr);
-    ~DEuler()
-    {
-        if (stream != nullptr)
-        {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
+    ~DEuler() = default;
 
     void sync() { SC_HIP(hipStreamSynchronize(stream)); }
 
@@ -1761,7 +1755,7 @@ struct DEuler
         bool any = false;
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            any = any || gsplit[gi] != 0;
+            any = any || gsplit[gi] == 1;
         }
         if (any)
         {
@@ -1769,17 +1763,53 @@ struct DEuler
         }
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            Group const & gr = groups[gi];
-            if (gsplit[gi] != 0)
+            run_form(p, groups[gi], gsplit[gi]);
+        }
+    }
+
+    // Form 0: the fused kernel; 1: the split pair; 2: the fused kernel
+    // compiled for three wavefronts per SIMD (2D cell types only).
+    static constexpr int NFORM = 3;
+
+    static bool form_exists(int32_t type, int form)
+    {
+        return form != 2 || type == CT_TRI || type == CT_QUAD;
+    }
+
+    void run_form(KP const & p, Group const & gr, int form)
+    {
+        if (form == 1)
+        {
+            run_group<M_SPLIT1>(p, gr, gr.n);
+            run_group<M_SPLIT2>(p, gr, gr.n);
+        }
+        else if (form == 2)
+        {
+            if (gr.type == CT_TRI)
             {
-                run_group<M_SPLIT1>(p, gr, gr.n);
-                run_group<M_SPLIT2>(p, gr, gr.n);
+                launch_w3<CT_TRI>(p, gr);
             }
             else
             {
-                run_group<M_FUSED>(p, gr, gr.n);
+                launch_w3<CT_QUAD>(p, gr);
             }
         }
+        else
+        {
+            run_group<M_FUSED>(p, gr, gr.n);
+        }
+    }
+
+    template <int T>
+    void launch_w3(KP const & p, Group const & gr)
+    {
+        if (gr.n <= 0)
+        {
+            return;
+        }
+        GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, gr.n};
+        k_cell<T, M_FUSED | M_W3><<<grid_for(gr.n, BLK), BLK, 0, stream>>>(p, g);
+        SC_HIP(hipGetLastError());
     }
 
     // The autotuner: the first substep of a march runs both forms of
@@ -1791,7 +1821,7 @@ struct DEuler
         update();
         KP const p = params();
         size_t const ng = groups.size();
-        std::vector<hipEvent_t> ev(2 + 4 * ng);
+        std::vector<hipEvent_t> ev(2 + (NFORM + 1) * ng);
         for (auto & x : ev)
         {
             SC_HIP(hipEventCreate(&x));
@@ -1799,55 +1829,82 @@ struct DEuler
         SC_HIP(hipEventRecord(ev[0], stream));
         calc_solt();
         SC_HIP(hipEventRecord(ev[1], stream));
+        // Every form of every group, each between its own event pair (the
+        // later forms rewrite the same outputs with the same values).
         for (size_t gi = 0; gi < ng; ++gi)
         {
             Group const & gr = groups[gi];
-            SC_HIP(hipEventRecord(ev[2 + 4 * gi], stream));
-            run_group<M_FUSED>(p, gr, gr.n);
-            SC_HIP(hipEventRecord(ev[3 + 4 * gi], stream));
-            run_group<M_SPLIT1>(p, gr, gr.n);
-            run_group<M_SPLIT2>(p, gr, gr.n);
-            SC_HIP(hipEventRecord(ev[4 + 4 * gi], stream));
-            SC_HIP(hipEventRecord(ev[5 + 4 * gi], stream));
+            for (int f = 0; f < NFORM; ++f)
+            {
+                SC_HIP(hipEventRecord(ev[2 + (NFORM + 1) * gi + f], stream));
+                if (form_exists(gr.type, f))
+                {
+                    run_form(p, gr, f);
+                }
+            }
+            SC_HIP(hipEventRecord(ev[2 + (NFORM + 1) * gi + NFORM], stream));
         }
-        SC_HIP(hipEventSynchronize(ev[ng > 0 ? 5 + 4 * (ng - 1) : 1]));
+        SC_HIP(hipEventSynchronize(ev[ng > 0 ? 2 + (NFORM + 1) * (ng - 1) + NFORM : 1]));
         float solt = 0.0f;
         SC_HIP(hipEventElapsedTime(&solt, ev[0], ev[1]));
-        std::vector<float> tf(ng), ts(ng);
+        std::vector<float> tm(ng * NFORM, 0.0f);
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            SC_HIP(hipEventElapsedTime(&tf[gi], ev[2 + 4 * gi], ev[3 + 4 * gi]));
-            SC_HIP(hipEventElapsedTime(&ts[gi], ev[3 + 4 * gi], ev[4 + 4 * gi]));
+            for (int f = 0; f < NFORM; ++f)
+            {
+                SC_HIP(hipEventElapsedTime(&tm[gi * NFORM + f], ev[2 + (NFORM + 1) * gi + f],
+                                           ev[3 + (NFORM + 1) * gi + f]));
+            }
         }
         for (auto & x : ev)
         {
             (void)hipEventDestroy(x);
         }
-        // Best subset of split groups (at most six groups).
-        double best = -1.0;
-        unsigned bestmask = 0;
-        for (unsigned mask = 0; mask < (1u << ng); ++mask)
+        // Best form per group; calc_solt counts once when any group runs
+        // the split pair (at most six groups: every assignment is tried).
+        size_t ncomb = 1;
+        for (size_t gi = 0; gi < ng; ++gi)
         {
-            double t = mask != 0 ? solt : 0.0;
+            ncomb *= NFORM;
+        }
+        double best = -1.0;
+        size_t bestc = 0;
+        for (size_t c = 0; c < ncomb; ++c)
+        {
+            double t = 0.0;
+            bool any = false;
+            bool ok = true;
+            size_t code = c;
             for (size_t gi = 0; gi < ng; ++gi)
             {
-                t += ((mask >> gi) & 1u) != 0 ? ts[gi] : tf[gi];
+                int const f = static_cast<int>(code % NFORM);
+                code /= NFORM;
+                ok = ok && form_exists(groups[gi].type, f);
+                any = any || f == 1;
+                t += tm[gi * NFORM + f];
             }
+            if (!ok)
+            {
+                continue;
+            }
+            t += any ? solt : 0.0;
             if (best < 0.0 || t < best)
             {
                 best = t;
-                bestmask = mask;
+                bestc = c;
             }
         }
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            gsplit[gi] = static_cast<char>((bestmask >> gi) & 1u);
+            gsplit[gi] = static_cast<char>(bestc % NFORM);
+            bestc /= NFORM;
         }
         tuned = 1;
     }
 
-    // Bit t set when the group of cell type t runs the split pair; -1
-    // before the first substep has tuned the forms.
+    // Bit t set when the group of cell type t runs the split pair, bit
+    // 16 + t when it runs the fused kernel at three wavefronts per SIMD;
+    // -1 before the first substep has tuned the forms.
     int64_t split_mask() const
     {
         if (tuned == 0)
@@ -1857,9 +1914,13 @@ struct DEuler
         int64_t m = 0;
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            if (gsplit[gi] != 0)
+            if (gsplit[gi] == 1)
             {
                 m |= int64_t(1) << groups[gi].type;
+            }
+            else if (gsplit[gi] == 2)
+            {
+                m |= int64_t(1) << (16 + groups[gi].type);
             }
         }
         return m;
@@ -1903,7 +1964,7 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
     {
         fail(SC_EVALUE, "EulerCore: ndim must be 2 or 3");
     }
-    SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    AllocScope const scope(stream);
     ndim = m->ndim;
     neq = ndim + 2;
     G = m->ngstcell;
@@ -2045,16 +2106,22 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         batch_set_dt(d.data());
     }
     // Fused or split per group: timed by the first substep unless
-    // SC_EULER_SPLIT fixes it (0 all fused, 1 all split, a: split in 3D).
+    // SC_EULER_SPLIT fixes it (0 all fused, 1 all split, a: split in 3D,
+    // 3: the three-wavefront fused kernel for 2D types, fused otherwise).
     gsplit.assign(groups.size(), 0);
     if (char const * env = std::getenv("SC_EULER_SPLIT"))
     {
         char const c = env[0];
-        if (c == '0' || c == '1' || c == 'a')
+        if (c == '0' || c == '1' || c == 'a' || c == '3')
         {
-            for (auto & x : gsplit)
+            for (size_t gi = 0; gi < groups.size(); ++gi)
             {
-                x = static_cast<char>(c == '1' || (c == 'a' && ndim == 3));
+                int f = (c == '1' || (c == 'a' && ndim == 3)) ? 1 : 0;
+                if (c == '3' && form_exists(groups[gi].type, 2))
+                {
+                    f = 2;
+                }
+                gsplit[gi] = static_cast<char>(f);
             }
             tuned = 1;
         }
@@ -2126,6 +2193,7 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
 // kernels read them per rank (hdt = dt/2, qdt = dt/4 as for one solve).
 void DEuler::batch_set_dt(double const * dt_inst)
 {
+    AllocScope const scope(stream);
     std::vector<double> h(static_cast<size_t>(ninst));
     std::vector<double> q(static_cast<size_t>(ninst));
     for (int64_t k = 0; k < ninst; ++k)
@@ -2142,6 +2210,7 @@ void DEuler::batch_set_dt(double const * dt_inst)
 // (from the current so0n), then set.
 void DEuler::batch_cfl_dt(double cfl, double * dt_inst)
 {
+    AllocScope const scope(stream);
     DevBuf<unsigned long long> cm;
     cm.alloc(ninst);
     SC_HIP(hipMemsetAsync(cm.ptr, 0, cm.bytes(), stream));
@@ -2185,6 +2254,7 @@ void DEuler::batch_cfl_dt(double cfl, double * dt_inst)
 // Largest calc_cfl number over the body cells at dt = 1, from so0n.
 double DEuler::cfl_max()
 {
+    AllocScope const scope(stream);
     DevBuf<unsigned long long> cm;
     cm.alloc(1);
     SC_HIP(hipMemsetAsync(cm.ptr, 0, cm.bytes(), stream));
@@ -2219,6 +2289,7 @@ double DEuler::cfl_max()
 
 void DEuler::prepare_ce()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     mesh->sync();
     MeshView mv;
@@ -2263,6 +2334,7 @@ void DEuler::prepare_ce()
 
 void DEuler::init_solution(double ga, double rho, double const * v, double p)
 {
+    AllocScope const scope(stream);
     // euler_solution.cpp:60-98
     if (ga <= 1.0)
     {
@@ -2294,6 +2366,7 @@ void DEuler::init_solution(double ga, double rho, double const * v, double p)
 // euler_bc.cpp:324-391
 void DEuler::add_bc(int32_t kind, std::vector<int32_t> const & faces, std::vector<double> const & value)
 {
+    AllocScope const scope(stream);
     int32_t const nface = mesh->nface;
     for (int32_t f : faces)
     {
@@ -2355,6 +2428,7 @@ void DEuler::add_bc(int32_t kind, std::vector<int32_t> const & faces, std::vecto
 // registration, exactly as the reference's in-order passes leave it.
 void DEuler::refresh_bc()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     fill<int32_t>(gkind.ptr, G, BC_NONE, s);
     fill<int32_t>(ginl.ptr, G, 0, s);
@@ -2429,6 +2503,7 @@ void DEuler::table(int id, int64_t & rows, int64_t & w, int64_t & ngh, int & ran
 
 void DEuler::copy(int id, double * host, int64_t count, bool down)
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     int64_t rows, w, ngh;
     int rank;
@@ -2555,6 +2630,7 @@ int sceuler_batch_set_dt(void * h, double const * dt)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst == 0)
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
@@ -2569,6 +2645,7 @@ int sceuler_batch_cfl_dt(void * h, double cfl, double * dt_out)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst == 0)
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
@@ -2585,6 +2662,7 @@ int sceuler_cfl_dt(void * h, double cfl, double * dt)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             double const v = e.cfl_max();
             if (!(v > 0.0))
             {
@@ -2600,6 +2678,7 @@ int sceuler_march_batch(void * h, double * ms_total)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst == 0)
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
@@ -2635,6 +2714,7 @@ void sceuler_destroy(void * h) { delete static_cast<DEuler *>(h); }
 int sceuler_counts(void * h, int64_t * out)
 {
     DEuler const & e = *static_cast<DEuler *>(h);
+    scamd::AllocScope const alloc_scope_(e.stream);
     out[0] = e.ndim;
     out[1] = e.N;
     out[2] = e.G;
@@ -2646,6 +2726,7 @@ int sceuler_counts(void * h, int64_t * out)
 double sceuler_get_param(void * h, int which)
 {
     DEuler const & e = *static_cast<DEuler *>(h);
+    scamd::AllocScope const alloc_scope_(e.stream);
     switch (which)
     {
     case 0: return e.sigma0;
@@ -2662,6 +2743,7 @@ int sceuler_set_param(void * h, int which, double v)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             switch (which)
             {
             case 0: e.sigma0 = v; break;
@@ -2713,6 +2795,7 @@ int sceuler_stage(void * h, int stage)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst > 0 && (stage == 3 || stage == 4 || stage == 5 || stage == 8))
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
@@ -2744,6 +2827,7 @@ int sceuler_march(void * h, int64_t steps)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst > 0)
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
@@ -2760,6 +2844,7 @@ int sceuler_march_async(void * h, int64_t steps)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst > 0)
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
@@ -2801,6 +2886,7 @@ int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * m
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (e.ninst > 0)
             {
                 scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver marches with march_batch");
@@ -2857,6 +2943,7 @@ int sceuler_init_solution(void * h, double gamma, double rho, double const * v, 
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (nv != e.ndim)
             {
                 scamd::fail(scamd::SC_EVALUE, "EulerCore::init_solution: velocity size " + std::to_string(nv)
@@ -2889,6 +2976,7 @@ int sceuler_normal_matrix(void * h, int32_t ifc, double * out)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             if (ifc < 0 || ifc >= e.mesh->nface)
             {
                 scamd::fail(scamd::SC_EINDEX, "EulerCore: face " + std::to_string(ifc) + " out of range");
@@ -2909,10 +2997,11 @@ int sceuler_reduce(void * h, double * out)
         [&]()
         {
             DEuler & e = *static_cast<DEuler *>(h);
+            scamd::AllocScope const alloc_scope_(e.stream);
             constexpr int NB = 256;
             scamd::DevBuf<double> part;
             part.alloc(NB * 3);
-            scamd::k_reduce<<<NB, scamd::BLK, 0, e.stream>>>(e.R, e.G, e.s0[1 - e.cur].ptr, e.mesh->clvol_b(),
+            scamd::k_reduce<<<NB, scamd::BLK, 0, e.stream>>>(e.R, e.G, e.neq, e.s0[1 - e.cur].ptr, e.mesh->clvol_b(),
                                                              e.ref_of_int.ptr, e.cflo.ptr, part.ptr);
             SC_HIP(hipGetLastError());
             std::vector<double> hp(NB * 3);

@@ -919,7 +919,7 @@ DMesh::DMesh(int nd, int32_t nn, int32_t nf, int32_t nc)
     {
         fail(SC_EVALUE, "StaticMesh: negative size");
     }
-    SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    AllocScope const scope(stream);
     // StaticMesh.hpp:290-311; tables the reference leaves uninitialised are
     // zero here.
     ndcrd.alloc(static_cast<int64_t>(nnode) * ndim);
@@ -952,14 +952,7 @@ DMesh::DMesh(int nd, int32_t nn, int32_t nf, int32_t nc)
     sync();
 }
 
-DMesh::~DMesh()
-{
-    if (stream != nullptr)
-    {
-        (void)hipStreamSynchronize(stream);
-        (void)hipStreamDestroy(stream);
-    }
-}
+DMesh::~DMesh() = default;
 
 void DMesh::table(int id, int64_t & rows, int64_t & cols, int64_t & ngh, bool & is_int, void *& ptr) const
 {
@@ -987,6 +980,7 @@ void DMesh::table(int id, int64_t & rows, int64_t & cols, int64_t & ngh, bool & 
 
 void DMesh::build_interior(bool do_metric, bool do_edge)
 {
+    AllocScope const scope(stream);
     build_faces_from_cells();
     if (do_metric)
     {
@@ -1000,6 +994,7 @@ void DMesh::build_interior(bool do_metric, bool do_edge)
 
 void DMesh::build_faces_from_cells()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     if (ngstcell != 0 || ngstface != 0 || ngstnode != 0)
     {
@@ -1106,6 +1101,7 @@ void DMesh::build_faces_from_cells()
 
 void DMesh::calc_metric()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     if (nface > 0 && (ndim == 2 || ndim == 3))
     {
@@ -1136,6 +1132,7 @@ void DMesh::calc_metric()
 
 void DMesh::build_edge()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     DevBuf<int32_t> coff;
     coff.alloc(nface + 1);
@@ -1205,6 +1202,7 @@ std::vector<int32_t> DMesh::fcjcl_of(std::vector<int32_t> const & faces)
 // StaticMesh_boundary.cpp:14-78
 void DMesh::add_bc(std::string const & name, std::vector<int32_t> const & faces)
 {
+    AllocScope const scope(stream);
     if (nbound != 0)
     {
         fail(SC_ERUNTIME, "StaticMesh::add_bc: boundary is already built");
@@ -1260,6 +1258,7 @@ void DMesh::add_bc(std::string const & name, std::vector<int32_t> const & faces)
 // StaticMesh_boundary.cpp:103-182
 void DMesh::build_boundary()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     if (nbound != 0)
     {
@@ -1337,6 +1336,7 @@ void DMesh::build_boundary()
 // StaticMesh_boundary.cpp:185-261
 void DMesh::build_ghost()
 {
+    AllocScope const scope(stream);
     hipStream_t const s = stream;
     if (ngstcell != 0)
     {
@@ -1415,6 +1415,7 @@ void scmesh_destroy(void * h) { delete static_cast<DMesh *>(h); }
 int scmesh_counts(void * h, int64_t * out)
 {
     DMesh const & m = *static_cast<DMesh *>(h);
+    scamd::AllocScope const alloc_scope_(m.stream);
     out[0] = m.ndim;
     out[1] = m.nnode;
     out[2] = m.nface;
@@ -1462,6 +1463,7 @@ static int mesh_copy(void * h, int id, void * host, int64_t nbytes, bool down)
         [&]()
         {
             DMesh & m = *static_cast<DMesh *>(h);
+            scamd::AllocScope const alloc_scope_(m.stream);
             int64_t rows, cols, ngh;
             bool is_int;
             void * p;
@@ -1532,6 +1534,7 @@ int scmesh_bc(void * h, int ibc, char * name, int nname, int32_t * facn0, int32_
         [&]()
         {
             DMesh const & m = *static_cast<DMesh *>(h);
+            scamd::AllocScope const alloc_scope_(m.stream);
             if (ibc < 0 || static_cast<size_t>(ibc) >= m.bcs.size())
             {
                 scamd::fail(scamd::SC_EINDEX, "StaticMesh::bc: index out of range");

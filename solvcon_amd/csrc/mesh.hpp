@@ -67,7 +67,8 @@ struct DMesh
     int32_t nnode = 0, nface = 0, ncell = 0;
     int32_t nbound = 0, ngstnode = 0, ngstface = 0, ngstcell = 0;
     int32_t nedge = 0;
-    hipStream_t stream = nullptr;
+    StreamHolder sh;                // declared before the buffers: freed last
+    hipStream_t stream = sh.s;
 
     DevBuf<double> ndcrd, fccnd, fcnml, fcara, clcnd, clvol;
     DevBuf<int32_t> fctpn, cltpn, clgrp, fcnds, fccls, clnds, clfcs, ednds, bndfcs;

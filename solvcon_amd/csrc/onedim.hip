@@ -403,14 +403,15 @@ struct Core1D
 {
     int64_t n = 0;
     double dt = 0.0;
-    hipStream_t stream = nullptr;
+    StreamHolder sh; // declared before the buffers: freed last
+    hipStream_t stream = sh.s;
     DevBuf<double> coord, cfl, so0, so1, gamma;
 
     Core1D(int64_t ncoord, double time_increment)
         : n(ncoord)
         , dt(time_increment)
     {
-        SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        AllocScope const scope(stream);
         coord.alloc(n);
         cfl.alloc(n);
         so0.alloc(n * 3);
@@ -424,14 +425,7 @@ struct Core1D
         fill_gamma(1.4);
     }
 
-    ~Core1D()
-    {
-        if (stream != nullptr)
-        {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
+    ~Core1D() = default;
 
     void fill_gamma(double g);
 
@@ -612,6 +606,7 @@ int sc1d_clone(void * h, void ** out)
         [&]()
         {
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             auto * b = new Core1D(a.n, a.dt);
             for (int w = 0; w < 5; ++w)
             {
@@ -629,6 +624,7 @@ void sc1d_destroy(void * h) { delete static_cast<Core1D *>(h); }
 int sc1d_info(void * h, int64_t * ncoord, double * time_increment)
 {
     auto & a = *static_cast<Core1D *>(h);
+    scamd::AllocScope const alloc_scope_(a.stream);
     *ncoord = a.n;
     *time_increment = a.dt;
     return 0;
@@ -651,6 +647,7 @@ int sc1d_upload(void * h, int which, double const * host, int64_t count)
         [&]()
         {
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             auto & b = a.buf(which);
             if (count != b.count)
             {
@@ -667,6 +664,7 @@ int sc1d_download(void * h, int which, double * host, int64_t count)
         [&]()
         {
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             auto & b = a.buf(which);
             if (count != b.count)
             {
@@ -697,6 +695,7 @@ int sc1d_march_half_so1(void * h, int alpha, int odd_plane)
                 scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
             }
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             a.half_so1(alpha, odd_plane != 0);
             a.sync();
         });
@@ -727,6 +726,7 @@ int sc1d_march_half(void * h, int alpha, int second)
                 scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
             }
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             a.half(alpha, second != 0);
             a.sync();
         });
@@ -742,6 +742,7 @@ int sc1d_march(void * h, int alpha, int64_t steps)
                 scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
             }
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             a.march(alpha, steps);
             a.sync();
         });
@@ -787,6 +788,7 @@ int sc1d_derived(void * h, int which, double * host_out)
         [&]()
         {
             auto & a = *static_cast<Core1D *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             if (which < 0 || which > 5)
             {
                 scamd::fail(scamd::SC_EVALUE, "unknown derived field");

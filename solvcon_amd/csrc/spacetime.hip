@@ -548,7 +548,8 @@ struct StCore
     int64_t ncelm = 0;
     int64_t nvar = 1;
     double dt = 0.0;
-    hipStream_t stream = nullptr;
+    StreamHolder sh; // declared before the buffers: freed last
+    hipStream_t stream = sh.s;
     DevBuf<double> xc, so0, so1, cfl;
 
     StCore(int k, int64_t n, int64_t nv, double time_increment)
@@ -558,7 +559,7 @@ struct StCore
         , nvar(nv)
         , dt(time_increment)
     {
-        SC_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        AllocScope const scope(stream);
         xc.alloc(nx);
         so0.alloc(nx * nvar);
         so1.alloc(nx * nvar);
@@ -569,14 +570,7 @@ struct StCore
         SC_HIP(hipMemsetAsync(cfl.ptr, 0, cfl.bytes(), stream));
     }
 
-    ~StCore()
-    {
-        if (stream != nullptr)
-        {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-    }
+    ~StCore() = default;
 
     void sync() { SC_HIP(hipStreamSynchronize(stream)); }
 
@@ -845,6 +839,7 @@ int scst_clone(void * h, void ** out)
         [&]()
         {
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             auto * b = new StCore(a.kind, a.nx, a.nvar, a.dt);
             for (int w = 0; w < 4; ++w)
             {
@@ -871,6 +866,7 @@ int scst_upload(void * h, int which, double const * host, int64_t count)
         [&]()
         {
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             auto & b = a.buf(which);
             if (count != b.count)
             {
@@ -887,6 +883,7 @@ int scst_download(void * h, int which, double * host, int64_t count)
         [&]()
         {
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             auto & b = a.buf(which);
             if (count != b.count)
             {
@@ -907,6 +904,7 @@ int scst_stage(void * h, int stage, int alpha, int odd_plane)
                 scamd::fail(scamd::SC_EVALUE, "alpha must be 0, 1 or 2");
             }
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             a.stage(stage, alpha, odd_plane != 0);
             a.sync();
         });
@@ -922,6 +920,7 @@ int scst_march(void * h, int alpha, int64_t steps)
                 scamd::fail(scamd::SC_EVALUE, "alpha must be 0, 1 or 2");
             }
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             a.march(alpha, steps);
             a.sync();
         });
@@ -951,6 +950,7 @@ int scst_eval(void * h, int what, int alpha, int64_t xindex, int64_t iv, int sto
         [&]()
         {
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             if (what < 0 || what > 7)
             {
                 scamd::fail(scamd::SC_EVALUE, "spacetime solver: unknown element value");
@@ -974,6 +974,7 @@ int scst_so0p(void * h, int64_t iv, int odd_plane, double * host_out, int64_t n)
         [&]()
         {
             auto & a = *static_cast<StCore *>(h);
+            scamd::AllocScope const alloc_scope_(a.stream);
             if (iv < 0 || iv >= a.nvar)
             {
                 scamd::fail(scamd::SC_EINDEX, "get_so0p(): out of nvar range");

@@ -176,14 +176,20 @@ class EulerCore(object):
 
     @property
     def kernel_forms(self):
-        """{cell type id: 'fused' | 'split'} chosen by the autotuner of
-        the first substep (None before it; no reference counterpart)."""
+        """{cell type id: 'fused' | 'fused3' | 'split'} chosen by the
+        autotuner of the first substep (None before it; no reference
+        counterpart); 'fused3' is the fused kernel compiled for three
+        wavefronts per SIMD."""
         m = int(L.lib().sceuler_get_param(self._h, 4))
         if m < 0:
             return None
         types = np.unique(self.mesh.cltpn.ndarray[self.mesh.ngstcell:])
-        return {int(t): 'split' if (m >> int(t)) & 1 else 'fused'
-                for t in types}
+        out = {}
+        for t in types:
+            t = int(t)
+            out[t] = 'split' if (m >> t) & 1 else \
+                'fused3' if (m >> (16 + t)) & 1 else 'fused'
+        return out
 
     sigma0 = _param(0)
     taumin = _param(1)

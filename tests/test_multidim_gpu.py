@@ -220,6 +220,51 @@ def test_march_bitwise(orc, name, sfc):
     assert np.array_equal(held, oe.so0n)
 
 
+@pytest.mark.parametrize('form', ['0', '1', '3', 'a'])
+@pytest.mark.parametrize('name', ['tri16', 'channel', 'mixed3d'])
+def test_march_every_kernel_form(orc, monkeypatch, name, form):
+    # The autotuner may pick any form per cell type; each fixed form gives
+    # the oracle's values (0 fused, 1 split pair, 3 three-wavefront fused
+    # kernel in 2D, a split in 3D).
+    print('case: march', name, 'form', form, flush=True)
+    monkeypatch.setenv('SC_EULER_SPLIT', form)
+    defn = _defs()[name]
+    g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
+    _riemann(ge, oe, o, seed=4)
+    faces = list(o.bndfcs[:, 0])
+    ge.add_nonrefl(faces)
+    oe.add_bc(orc.BC_NONREFL, faces)
+    ge.bc_soln()
+    ge.bc_dsoln()
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    ge.march(7)
+    oe.march(7)
+    pair.same_euler(ge, oe)
+    forms = ge.kernel_forms
+    want = {'0': 'fused', '1': 'split'}.get(form)
+    for t, f in forms.items():
+        if want is not None:
+            assert f == want
+        elif form == '3':
+            assert f == ('fused3' if t in (3, 4) else 'fused')
+        else:
+            assert f == ('split' if t >= 5 else 'fused')
+
+
+def test_autotuner_reports_forms(orc, monkeypatch):
+    print('case: autotuner forms', flush=True)
+    monkeypatch.delenv('SC_EULER_SPLIT', raising=False)
+    defn = _defs()['mixed3d']
+    g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
+    assert ge.kernel_forms is None
+    ge.init_solution(GAMMA, 1.0, [0.1, 0.0, 0.0], 1.0)
+    ge.march(1)
+    assert set(ge.kernel_forms) == set(int(t) for t in
+                                      np.unique(g.cltpn.ndarray[g.ngstcell:]))
+    assert set(ge.kernel_forms.values()) <= {'fused', 'split', 'fused3'}
+
+
 def test_fused_equals_stages(orc):
     print('case: fused substep vs stages', flush=True)
     defn = _defs()['tri16']
