@@ -206,7 +206,8 @@ def run_mesh(args, which):
     b_sub = sum(b_substep(t) * n for t, n in mix.items())  # bytes/substep
     achieved = b_sub / (sub_ms * 1e-3) / 1e9
     log('bench: fused substep kernels %.4f ms avg (%d substeps), %.1f GB/s '
-        'algorithmic' % (sub_ms, 2 * nprof, achieved))
+        'algorithmic; whole march by events %.4f ms per step'
+        % (sub_ms, 2 * nprof, achieved, tot_ms / nprof))
     copy_gbs = None
     if args.copy_probe > 0:
         import ctypes

@@ -169,6 +169,10 @@ class Mirror(object):
     def before(self):
         if self.handed and self.base is not None and self.base.size:
             self._up(self.base)
+            # Device and host now agree; a view released since it was handed
+            # out cannot write again, so later calls need no upload.
+            if not self.held():
+                self.handed = False
 
     def after(self, written=True):
         if not written:

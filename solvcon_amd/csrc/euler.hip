@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <cmath>
@@ -84,9 +85,70 @@ enum Mode : int
     M_DSOLN = 8,
     M_STMEM = 16,
     M_BATCH = 32,
-    M_W3 = 64, // compile for three wavefronts per SIMD (registers capped at 168)
+    M_W3 = 64,  // compiled for three wavefronts per SIMD (at most 168 registers)
+    M_W2 = 128, // two wavefronts per SIMD (at most 256 registers)
+    M_W4 = 256, // four wavefronts per SIMD (at most 128 registers)
+    M_WAVES = M_W2 | M_W3 | M_W4,
+    M_STAGE = 512, // stage the block's own rows in LDS for neighbour reads
     M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
 };
+
+// Minimum wavefronts per SIMD a k_cell variant is compiled for; the
+// register budget per lane is 512 / waves.
+constexpr int waves_of(int mode)
+{
+    return (mode & M_W4) != 0 ? 4 : (mode & M_W3) != 0 ? 3 : (mode & M_W2) != 0 ? 2 : SC_CELL_WAVES;
+}
+
+// Kernel roles: 0 the fused kernel, 1 soln + BC + CFL, 2 dsoln.  Each
+// role is compiled in a few variants: a minimum wave count per SIMD (which
+// caps registers) and whether the block's own rows are staged in LDS.  A
+// variant code is waves + 10 * staged.
+constexpr int NVAR = 3;
+
+constexpr int mode_waves(int waves)
+{
+    return waves == 4 ? M_W4 : waves == 3 ? M_W3 : waves == 2 ? M_W2 : 0;
+}
+
+constexpr int role_variant(int nd, int role, int v)
+{
+    // 0 marks an unused slot.
+    if (nd == 2)
+    {
+        return role == 0 ? (v == 0 ? 1 : v == 1 ? 3 : 11) : (v == 0 ? 1 : 0);
+    }
+    return role == 0 ? (v == 0 ? 1 : v == 1 ? 2 : 0) : role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : 12);
+}
+
+constexpr int variant_mode(int code)
+{
+    return mode_waves(code % 10) | (code >= 10 ? M_STAGE : 0);
+}
+
+constexpr bool variant_built(int nd, int mode)
+{
+    if ((mode & (M_WAVES | M_STAGE)) == 0)
+    {
+        return true;
+    }
+    if ((mode & M_BATCH) != 0)
+    {
+        return false;
+    }
+    bool const soln = (mode & M_SOLN) != 0;
+    bool const dsoln = (mode & M_DSOLN) != 0;
+    int const role = soln && dsoln ? 0 : soln ? 1 : 2;
+    for (int v = 0; v < NVAR; ++v)
+    {
+        int const c = role_variant(nd, role, v);
+        if (c != 0 && variant_mode(c) == (mode & (M_WAVES | M_STAGE)))
+        {
+            return true;
+        }
+    }
+    return false;
+}
 
 struct KP
 {
@@ -150,8 +212,11 @@ void fill(T * a, int64_t n, T v, hipStream_t s)
 // The per-cell march kernel.
 // --------------------------------------------------------------------------
 
+// Select between two values that have both been loaded (no branch).
+__device__ inline double pick(bool first, double a, double b) { return first ? a : b; }
+
 template <int T, int MODE>
-__global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k_cell(KP const p, GV const g)
+__global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV const g)
 {
     using TR = Traits<T>;
     using LY = Layout<T>;
@@ -160,7 +225,54 @@ __global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k
     constexpr int NEQ = ND + 2;
     constexpr int NFGE = TR::NFGE;
 
-    int64_t const i = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+    int64_t const i0 = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x;
+    int64_t const i = i0 + threadIdx.x;
+
+    // The block's own rows of the current level (so0c, so1c, so0t, gamma,
+    // solution points) are staged in LDS with coalesced loads; a neighbour
+    // that is one of them is then read from LDS instead of being gathered
+    // from global memory.  In SFC order most neighbours are.
+    constexpr bool STAGE = (MODE & M_STAGE) != 0 && (MODE & (M_SOLN | M_DSOLN)) != 0;
+    constexpr bool STG_ST = ((MODE & M_SOLN) != 0 && (MODE & M_STMEM) != 0) || (MODE & (M_SOLN | M_DSOLN)) == M_DSOLN;
+    constexpr bool STG_GM = (MODE & M_SOLN) != 0;
+    constexpr int O_S1 = BLK * NEQ;
+    constexpr int O_ST = O_S1 + BLK * NEQ * ND;
+    constexpr int O_GM = O_ST + (STG_ST ? BLK * NEQ : 0);
+    constexpr int O_CN = O_GM + (STG_GM ? BLK : 0);
+    constexpr int LW = STAGE ? O_CN + BLK * ND : 1;
+    __shared__ double lds[LW];
+    int64_t const rbase = p.G + g.off + i0;
+    int const nrow = static_cast<int>(g.cnt - i0 < BLK ? g.cnt - i0 : BLK);
+    if constexpr (STAGE)
+    {
+        for (int q = threadIdx.x; q < nrow * NEQ; q += BLK)
+        {
+            lds[q] = p.s0c[rbase * NEQ + q];
+        }
+        for (int q = threadIdx.x; q < nrow * NEQ * ND; q += BLK)
+        {
+            lds[O_S1 + q] = p.s1c[rbase * (NEQ * ND) + q];
+        }
+        if constexpr (STG_ST)
+        {
+            for (int q = threadIdx.x; q < nrow * NEQ; q += BLK)
+            {
+                lds[O_ST + q] = p.st[rbase * NEQ + q];
+            }
+        }
+        if constexpr (STG_GM)
+        {
+            for (int q = threadIdx.x; q < nrow; q += BLK)
+            {
+                lds[O_GM + q] = p.gam[rbase + q];
+            }
+        }
+        for (int q = threadIdx.x; q < nrow * ND; q += BLK)
+        {
+            lds[O_CN + q] = p.cen[rbase * ND + q];
+        }
+        __syncthreads();
+    }
     if (i >= g.cnt)
     {
         return;
@@ -178,20 +290,30 @@ __global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k
     }
 
     int64_t nb[NF];
+    int lr[NF]; // the neighbour's row in the LDS copy, or -1
 #pragma unroll
     for (int f = 0; f < NF; ++f)
     {
         nb[f] = g.nbr[f * n + i];
+        lr[f] = (STAGE && nb[f] >= rbase && nb[f] < rbase + nrow) ? static_cast<int>(nb[f] - rbase) : -1;
     }
     // Neighbour solution points: CE centroid (body) or mesh centroid (ghost).
+    // A neighbour value is read from both places with no branch: from LDS
+    // at its staged row (row 0 when not staged) and from global memory at
+    // its row, where every staged lane reads the block's first row instead;
+    // the lanes of one load then touch only the cache lines of the
+    // neighbours that are not staged.
     double jce[NF][ND];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
     {
+        bool const inb = lr[f] >= 0;
+        double const * cg = p.cen + (inb ? rbase : nb[f]) * ND;
+        double const * cl = lds + O_CN + (inb ? lr[f] : 0) * ND;
 #pragma unroll
         for (int d = 0; d < ND; ++d)
         {
-            jce[f][d] = p.cen[nb[f] * ND + d];
+            jce[f][d] = pick(inb, cl[d], cg[d]);
         }
     }
     double st[NF][NEQ];
@@ -206,22 +328,33 @@ __global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k
         {
             acc[k] = 0.0;
         }
-#pragma unroll
-        for (int f = 0; f < NF; ++f)
-        {
+        static_for<0, NF>([&](auto fc) {
+            constexpr int f = decltype(fc)::value;
             int64_t const j = nb[f];
             double js[NEQ];
             double j1[NEQ][ND];
+            bool const inb = lr[f] >= 0;
+            int const l = inb ? lr[f] : 0;
+            int64_t const jg = inb ? rbase : j;
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                js[k] = p.s0c[(j) * NEQ + k];
+                js[k] = pick(inb, lds[l * NEQ + k], p.s0c[jg * NEQ + k]);
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
-                    j1[k][d] = p.s1c[(j) * (NEQ * ND) + k * ND + d];
+                    j1[k][d] = pick(inb, lds[O_S1 + l * (NEQ * ND) + k * ND + d], p.s1c[jg * (NEQ * ND) + k * ND + d]);
                 }
             }
+            if constexpr ((MODE & M_STMEM) != 0)
+            {
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    st[f][k] = pick(inb, lds[O_ST + l * NEQ + k], p.st[jg * NEQ + k]);
+                }
+            }
+            double const jgam = pick(inb, lds[O_GM + l], p.gam[jg]);
             double const bvol = geo[(LY::FV + 1 + f) * n];
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
@@ -234,54 +367,105 @@ __global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k
                 }
                 acc[k] += fusp * bvol;
             }
-            typename JacOf<ND>::type jac;
-            jac.update(p.gam[j], js);
-            if constexpr ((MODE & M_STMEM) != 0)
+            JacS<ND> jac;
+            jac.set(jgam, js);
+            if constexpr ((MODE & M_STMEM) == 0)
             {
+                solt_of<ND>(jac, j1, st[f]);
+            }
+            constexpr int NS = TR::nsub(f);
+            if constexpr (ND == 2)
+            {
+                // The whole Jacobian stays in registers; sub-face by
+                // sub-face, every equation.
+                double jf[NEQ][NEQ][ND];
+                double ff[NEQ][ND];
+                static_for<0, NEQ>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    jac.template row<k>(jf[k], ff[k]);
+                });
 #pragma unroll
-                for (int k = 0; k < NEQ; ++k)
+                for (int s = 0; s < NS; ++s)
                 {
-                    st[f][k] = p.st[(j) * NEQ + k];
+                    int const sf = TR::soff(f) + s;
+                    double usfc[NEQ];
+#pragma unroll
+                    for (int k = 0; k < NEQ; ++k)
+                    {
+                        usfc[k] = qdt * st[f][k];
+#pragma unroll
+                        for (int d = 0; d < ND; ++d)
+                        {
+                            usfc[k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < NEQ; ++k)
+                    {
+                        double dot = 0.0;
+#pragma unroll
+                        for (int d2 = 0; d2 < ND; ++d2)
+                        {
+                            double dfcn = ff[k][d2];
+#pragma unroll
+                            for (int m = 0; m < NEQ; ++m)
+                            {
+                                dfcn += jf[k][m][d2] * usfc[m];
+                            }
+                            dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
+                        }
+                        acc[k] -= hdt * dot;
+                    }
                 }
             }
             else
             {
-                solt_of<ND>(jac, j1, st[f]);
-            }
+                // The solution at every sub-face centroid of this face, then
+                // the flux through each sub-face one Jacobian row (equation)
+                // at a time, so only one row is live; acc[k] still receives
+                // its sub-face terms in sub-face order.
+                double usfc[NS][NEQ];
 #pragma unroll
-            for (int s = 0; s < TR::nsub(f); ++s)
-            {
-                int const sf = TR::soff(f) + s;
-                double usfc[NEQ];
-#pragma unroll
-                for (int k = 0; k < NEQ; ++k)
+                for (int s = 0; s < NS; ++s)
                 {
-                    usfc[k] = qdt * st[f][k];
+                    int const sf = TR::soff(f) + s;
 #pragma unroll
-                    for (int d = 0; d < ND; ++d)
+                    for (int k = 0; k < NEQ; ++k)
                     {
-                        usfc[k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
-                    }
-                }
+                        usfc[s][k] = qdt * st[f][k];
 #pragma unroll
-                for (int k = 0; k < NEQ; ++k)
-                {
-                    double dot = 0.0;
-#pragma unroll
-                    for (int d2 = 0; d2 < ND; ++d2)
-                    {
-                        double dfcn = jac.f[k][d2];
-#pragma unroll
-                        for (int m = 0; m < NEQ; ++m)
+                        for (int d = 0; d < ND; ++d)
                         {
-                            dfcn += jac.j[k][m][d2] * usfc[m];
+                            usfc[s][k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
                         }
-                        dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
                     }
-                    acc[k] -= hdt * dot;
                 }
+                static_for<0, NEQ>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    double jr[NEQ][ND];
+                    double fr[ND];
+                    jac.template row<k>(jr, fr);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                    {
+                        int const sf = TR::soff(f) + s;
+                        double dot = 0.0;
+#pragma unroll
+                        for (int d2 = 0; d2 < ND; ++d2)
+                        {
+                            double dfcn = fr[d2];
+#pragma unroll
+                            for (int m = 0; m < NEQ; ++m)
+                            {
+                                dfcn += jr[m][d2] * usfc[s][m];
+                            }
+                            dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
+                        }
+                        acc[k] -= hdt * dot;
+                    }
+                });
             }
-        }
+        });
         double const cvol = geo[LY::FV * n];
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
@@ -407,10 +591,13 @@ __global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k
 #pragma unroll
             for (int f = 0; f < NF; ++f)
             {
+                bool const inb = lr[f] >= 0;
+                int const l = inb ? lr[f] : 0;
+                int64_t const jg = inb ? rbase : nb[f];
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    st[f][k] = p.st[nb[f] * NEQ + k];
+                    st[f][k] = pick(inb, lds[O_ST + l * NEQ + k], p.st[jg * NEQ + k]);
                 }
             }
         }
@@ -514,14 +701,17 @@ __global__ void __launch_bounds__(BLK, (MODE & M_W3) != 0 ? 3 : SC_CELL_WAVES) k
                 jdis[d] = (gp[f][d] - jce[f][d]) + ice[d] - cnd[d];
             }
             int64_t const j = nb[f];
+            bool const inb = lr[f] >= 0;
+            int const l = inb ? lr[f] : 0;
+            int64_t const jg = inb ? rbase : j;
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                double val = p.s0c[(j) * NEQ + k] + hdt * st[f][k] - u0[k];
+                double val = pick(inb, lds[l * NEQ + k], p.s0c[jg * NEQ + k]) + hdt * st[f][k] - u0[k];
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
-                    val += jdis[d] * p.s1c[(j) * (NEQ * ND) + k * ND + d];
+                    val += jdis[d] * pick(inb, lds[O_S1 + l * (NEQ * ND) + k * ND + d], p.s1c[jg * (NEQ * ND) + k * ND + d]);
                 }
                 udf[f][k] = val;
             }
@@ -671,8 +861,8 @@ __global__ void k_solt(KP const p)
             s1[k][d] = p.s1c[(r) * (NEQ * ND) + k * ND + d];
         }
     }
-    typename JacOf<ND>::type jac;
-    jac.update(p.gam[r], s0);
+    JacS<ND> jac;
+    jac.set(p.gam[r], s0);
     double st[NEQ];
     solt_of<ND>(jac, s1, st);
 #pragma unroll
@@ -1516,8 +1706,18 @@ struct DEuler
     double taumin = 0.0;
     double tauscale = 1.0;
     bool use_sfc = true;
-    int tuned = 0;              // 0 until the first substep has timed both forms
-    std::vector<char> gsplit;   // per group: 0 fused, 1 split pair, 2 fused at 3 waves
+    // Kernel form of a type group: the fused kernel in variant vf, or the
+    // split pair (soln + BC + CFL, then dsoln) in variants v1 and v2 (codes
+    // of role_variant).
+    struct Form
+    {
+        int split = 0;
+        int vf = 1;
+        int v1 = 1;
+        int v2 = 1;
+    };
+    int tuned = 0;            // 0 until the first substep has timed the forms
+    std::vector<Form> gform;  // per group
     StreamHolder sh;            // declared before the buffers: freed last
     hipStream_t stream = sh.s;
 
@@ -1585,27 +1785,57 @@ struct DEuler
         return p;
     }
 
-    // One kernel of mode MODE over the first cnt cells of a type group.
+    template <int T, int MODE>
+    void launch_cell(KP const & p, GV const & g, unsigned nb)
+    {
+        if constexpr (variant_built(Traits<T>::ND, MODE))
+        {
+            k_cell<T, MODE><<<nb, BLK, 0, stream>>>(p, g);
+        }
+        else
+        {
+            fail(SC_ERUNTIME, "EulerCore: kernel variant not built for this cell type");
+        }
+    }
+
+    // One kernel of mode MODE over the first cnt cells of a type group; a
+    // negative cnt launches one block that processes no cell, which loads
+    // the kernel's code so a later timed launch does not pay for it.
     template <int MODE>
     void run_group(KP const & p, Group const & gr, int64_t cnt)
     {
-        if (cnt <= 0)
+        if (cnt == 0)
         {
             return;
         }
-        GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, cnt};
-        unsigned const nb = grid_for(cnt, BLK);
+        GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, cnt < 0 ? 0 : cnt};
+        unsigned const nb = cnt < 0 ? 1u : grid_for(cnt, BLK);
         switch (gr.type)
         {
-        case CT_TRI: k_cell<CT_TRI, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-        case CT_QUAD: k_cell<CT_QUAD, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-        case CT_TET: k_cell<CT_TET, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-        case CT_HEX: k_cell<CT_HEX, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-        case CT_PRISM: k_cell<CT_PRISM, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
-        case CT_PYRAMID: k_cell<CT_PYRAMID, MODE><<<nb, BLK, 0, stream>>>(p, g); break;
+        case CT_TRI: launch_cell<CT_TRI, MODE>(p, g, nb); break;
+        case CT_QUAD: launch_cell<CT_QUAD, MODE>(p, g, nb); break;
+        case CT_TET: launch_cell<CT_TET, MODE>(p, g, nb); break;
+        case CT_HEX: launch_cell<CT_HEX, MODE>(p, g, nb); break;
+        case CT_PRISM: launch_cell<CT_PRISM, MODE>(p, g, nb); break;
+        case CT_PYRAMID: launch_cell<CT_PYRAMID, MODE>(p, g, nb); break;
         default: fail(SC_ERUNTIME, "EulerCore: unsupported cell type in group");
         }
         SC_HIP(hipGetLastError());
+    }
+
+    // run_group over a whole group with the variant chosen at run time.
+    template <int MODE>
+    void run_group_v(KP const & p, Group const & gr, int code, bool warm = false)
+    {
+        int64_t const cnt = warm ? -1 : gr.n;
+        switch (code)
+        {
+        case 2: run_group<MODE | variant_mode(2)>(p, gr, cnt); break;
+        case 3: run_group<MODE | variant_mode(3)>(p, gr, cnt); break;
+        case 11: run_group<MODE | variant_mode(11)>(p, gr, cnt); break;
+        case 12: run_group<MODE | variant_mode(12)>(p, gr, cnt); break;
+        default: run_group<MODE>(p, gr, cnt); break;
+        }
     }
 
     template <int MODE>
@@ -1738,8 +1968,7 @@ struct DEuler
     // split pair that reads so0t from memory: calc_solt over every row
     // (once per substep when any group is split), then soln + BC + CFL,
     // then dsoln.  Both forms compute the same values; the split pair
-    // needs fewer registers per kernel and wins where the fused kernel
-    // runs at one wavefront per SIMD.
+    // needs fewer registers per kernel.
     static constexpr int M_SPLIT1 = M_SOLN | M_BC | M_CFL | M_STMEM;
     static constexpr int M_SPLIT2 = M_DSOLN | M_STMEM;
 
@@ -1753,9 +1982,9 @@ struct DEuler
         update();
         KP const p = params();
         bool any = false;
-        for (size_t gi = 0; gi < groups.size(); ++gi)
+        for (auto const & f : gform)
         {
-            any = any || gsplit[gi] == 1;
+            any = any || f.split != 0;
         }
         if (any)
         {
@@ -1763,167 +1992,203 @@ struct DEuler
         }
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            run_form(p, groups[gi], gsplit[gi]);
+            run_form(p, groups[gi], gform[gi]);
         }
     }
 
-    // Form 0: the fused kernel; 1: the split pair; 2: the fused kernel
-    // compiled for three wavefronts per SIMD (2D cell types only).
-    static constexpr int NFORM = 3;
-
-    static bool form_exists(int32_t type, int form)
+    void run_role(KP const & p, Group const & gr, int role, int code, bool warm = false)
     {
-        return form != 2 || type == CT_TRI || type == CT_QUAD;
-    }
-
-    void run_form(KP const & p, Group const & gr, int form)
-    {
-        if (form == 1)
+        if (role == 0)
         {
-            run_group<M_SPLIT1>(p, gr, gr.n);
-            run_group<M_SPLIT2>(p, gr, gr.n);
+            run_group_v<M_FUSED>(p, gr, code, warm);
         }
-        else if (form == 2)
+        else if (role == 1)
         {
-            if (gr.type == CT_TRI)
-            {
-                launch_w3<CT_TRI>(p, gr);
-            }
-            else
-            {
-                launch_w3<CT_QUAD>(p, gr);
-            }
+            run_group_v<M_SPLIT1>(p, gr, code, warm);
         }
         else
         {
-            run_group<M_FUSED>(p, gr, gr.n);
+            run_group_v<M_SPLIT2>(p, gr, code, warm);
         }
     }
 
-    template <int T>
-    void launch_w3(KP const & p, Group const & gr)
+    void run_form(KP const & p, Group const & gr, Form const & f)
     {
-        if (gr.n <= 0)
+        if (f.split != 0)
         {
-            return;
+            run_role(p, gr, 1, f.v1);
+            run_role(p, gr, 2, f.v2);
         }
-        GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, gr.n};
-        k_cell<T, M_FUSED | M_W3><<<grid_for(gr.n, BLK), BLK, 0, stream>>>(p, g);
-        SC_HIP(hipGetLastError());
+        else
+        {
+            run_role(p, gr, 0, f.vf);
+        }
     }
 
-    // The autotuner: the first substep of a march runs both forms of
-    // every group (the split pair rewrites the same outputs with the same
-    // values), times them with HIP events, and keeps for each group the
-    // faster form, counting calc_solt once if any group is split.
+    // The autotuner: the first substep of a march runs every compiled
+    // variant of every kernel role of every group (later variants rewrite
+    // the same outputs with the same values), times each with HIP events,
+    // and keeps per group the fastest fused variant or the fastest split1 +
+    // split2 pair, counting calc_solt once if any group is split.
     void tune_substep()
     {
         update();
         KP const p = params();
         size_t const ng = groups.size();
-        std::vector<hipEvent_t> ev(2 + (NFORM + 1) * ng);
+        size_t const nslot = ng * 3 * NVAR;
+        std::vector<hipEvent_t> ev(2 + 2 * nslot, nullptr);
         for (auto & x : ev)
         {
             SC_HIP(hipEventCreate(&x));
         }
+        // Load every variant's code first (one empty block each).
+        calc_solt();
+        for (size_t gi = 0; gi < ng; ++gi)
+        {
+            for (int role = 0; role < 3; ++role)
+            {
+                for (int v = 0; v < NVAR; ++v)
+                {
+                    int const c = role_variant(ndim, role, v);
+                    if (c != 0)
+                    {
+                        run_role(p, groups[gi], role, c, true);
+                    }
+                }
+            }
+        }
         SC_HIP(hipEventRecord(ev[0], stream));
         calc_solt();
         SC_HIP(hipEventRecord(ev[1], stream));
-        // Every form of every group, each between its own event pair (the
-        // later forms rewrite the same outputs with the same values).
+        std::vector<char> ran(nslot, 0);
+        hipEvent_t last = ev[1];
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            Group const & gr = groups[gi];
-            for (int f = 0; f < NFORM; ++f)
+            for (int role = 0; role < 3; ++role)
             {
-                SC_HIP(hipEventRecord(ev[2 + (NFORM + 1) * gi + f], stream));
-                if (form_exists(gr.type, f))
+                for (int v = 0; v < NVAR; ++v)
                 {
-                    run_form(p, gr, f);
+                    int const c = role_variant(ndim, role, v);
+                    if (c == 0)
+                    {
+                        continue;
+                    }
+                    size_t const sl = (gi * 3 + role) * NVAR + v;
+                    SC_HIP(hipEventRecord(ev[2 + 2 * sl], stream));
+                    run_role(p, groups[gi], role, c);
+                    SC_HIP(hipEventRecord(ev[3 + 2 * sl], stream));
+                    ran[sl] = 1;
+                    last = ev[3 + 2 * sl];
                 }
             }
-            SC_HIP(hipEventRecord(ev[2 + (NFORM + 1) * gi + NFORM], stream));
         }
-        SC_HIP(hipEventSynchronize(ev[ng > 0 ? 2 + (NFORM + 1) * (ng - 1) + NFORM : 1]));
+        SC_HIP(hipEventSynchronize(last));
         float solt = 0.0f;
         SC_HIP(hipEventElapsedTime(&solt, ev[0], ev[1]));
-        std::vector<float> tm(ng * NFORM, 0.0f);
-        for (size_t gi = 0; gi < ng; ++gi)
+        std::vector<float> tm(nslot, -1.0f);
+        for (size_t sl = 0; sl < nslot; ++sl)
         {
-            for (int f = 0; f < NFORM; ++f)
+            if (ran[sl] != 0)
             {
-                SC_HIP(hipEventElapsedTime(&tm[gi * NFORM + f], ev[2 + (NFORM + 1) * gi + f],
-                                           ev[3 + (NFORM + 1) * gi + f]));
+                SC_HIP(hipEventElapsedTime(&tm[sl], ev[2 + 2 * sl], ev[3 + 2 * sl]));
             }
         }
         for (auto & x : ev)
         {
             (void)hipEventDestroy(x);
         }
-        // Best form per group; calc_solt counts once when any group runs
-        // the split pair (at most six groups: every assignment is tried).
-        size_t ncomb = 1;
-        for (size_t gi = 0; gi < ng; ++gi)
+        if (std::getenv("SC_EULER_TUNE_LOG") != nullptr)
         {
-            ncomb *= NFORM;
-        }
-        double best = -1.0;
-        size_t bestc = 0;
-        for (size_t c = 0; c < ncomb; ++c)
-        {
-            double t = 0.0;
-            bool any = false;
-            bool ok = true;
-            size_t code = c;
+            std::fprintf(stderr, "EulerCore tuning: calc_solt %.4f ms\n", solt);
             for (size_t gi = 0; gi < ng; ++gi)
             {
-                int const f = static_cast<int>(code % NFORM);
-                code /= NFORM;
-                ok = ok && form_exists(groups[gi].type, f);
-                any = any || f == 1;
-                t += tm[gi * NFORM + f];
+                for (int role = 0; role < 3; ++role)
+                {
+                    std::fprintf(stderr, "  type %d cells %lld %s", groups[gi].type,
+                                 static_cast<long long>(groups[gi].n),
+                                 role == 0 ? "fused " : role == 1 ? "split1" : "split2");
+                    for (int v = 0; v < NVAR; ++v)
+                    {
+                        float const t = tm[(gi * 3 + role) * NVAR + v];
+                        int const c = role_variant(ndim, role, v);
+                        if (t >= 0.0f)
+                        {
+                            std::fprintf(stderr, "  w%d%s %.4f ms", c % 10, c >= 10 ? "+lds" : "", t);
+                        }
+                    }
+                    std::fprintf(stderr, "\n");
+                }
             }
-            if (!ok)
+        }
+        std::vector<Form> fused(ng), split(ng);
+        std::vector<double> tf(ng, 0.0), ts(ng, 0.0);
+        for (size_t gi = 0; gi < ng; ++gi)
+        {
+            double best[3] = {-1.0, -1.0, -1.0};
+            int bc[3] = {1, 1, 1};
+            for (int role = 0; role < 3; ++role)
             {
-                continue;
+                for (int v = 0; v < NVAR; ++v)
+                {
+                    float const t = tm[(gi * 3 + role) * NVAR + v];
+                    if (t >= 0.0f && (best[role] < 0.0 || t < best[role]))
+                    {
+                        best[role] = t;
+                        bc[role] = role_variant(ndim, role, v);
+                    }
+                }
             }
-            t += any ? solt : 0.0;
-            if (best < 0.0 || t < best)
-            {
-                best = t;
-                bestc = c;
-            }
+            fused[gi].vf = bc[0];
+            split[gi].split = 1;
+            split[gi].v1 = bc[1];
+            split[gi].v2 = bc[2];
+            tf[gi] = best[0];
+            ts[gi] = best[1] + best[2];
+        }
+        double all_fused = 0.0;
+        double mixed = solt;
+        for (size_t gi = 0; gi < ng; ++gi)
+        {
+            all_fused += tf[gi];
+            mixed += std::min(tf[gi], ts[gi]);
         }
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            gsplit[gi] = static_cast<char>(bestc % NFORM);
-            bestc /= NFORM;
+            gform[gi] = (mixed < all_fused && ts[gi] < tf[gi]) ? split[gi] : fused[gi];
         }
         tuned = 1;
     }
 
-    // Bit t set when the group of cell type t runs the split pair, bit
-    // 16 + t when it runs the fused kernel at three wavefronts per SIMD;
-    // -1 before the first substep has tuned the forms.
-    int64_t split_mask() const
+    // Form of the group of one cell type: the fused variant code vf, or
+    // 10000 + 100 v1 + v2 for the split pair; 0 when the mesh has no such
+    // cells, -1 before the first substep has tuned.
+    int form_code(int32_t type) const
     {
         if (tuned == 0)
         {
             return -1;
         }
-        int64_t m = 0;
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            if (gsplit[gi] == 1)
+            if (groups[gi].type == type)
             {
-                m |= int64_t(1) << groups[gi].type;
-            }
-            else if (gsplit[gi] == 2)
-            {
-                m |= int64_t(1) << (16 + groups[gi].type);
+                Form const & f = gform[gi];
+                return f.split != 0 ? 10000 + 100 * f.v1 + f.v2 : f.vf;
             }
         }
-        return m;
+        return 0;
+    }
+
+    bool has_variant(int role, int code) const
+    {
+        for (int v = 0; v < NVAR; ++v)
+        {
+            if (role_variant(ndim, role, v) == code)
+            {
+                return true;
+            }
+        }
+        return false;
     }
 
     void march(int64_t steps)
@@ -2105,23 +2370,43 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         std::vector<double> d(static_cast<size_t>(ninst), time_increment);
         batch_set_dt(d.data());
     }
-    // Fused or split per group: timed by the first substep unless
-    // SC_EULER_SPLIT fixes it (0 all fused, 1 all split, a: split in 3D,
-    // 3: the three-wavefront fused kernel for 2D types, fused otherwise).
-    gsplit.assign(groups.size(), 0);
-    if (char const * env = std::getenv("SC_EULER_SPLIT"))
+    // The form of every group is timed by the first substep unless
+    // SC_EULER_FORM fixes it: fV (the fused kernel in variant V) or sA/B
+    // (the split pair in variants A and B), where a variant is the minimum
+    // wavefronts per SIMD followed by 's' when LDS-staged (f1, f3, f1s,
+    // s1s/2); groups whose type lacks a variant run f1.
+    gform.assign(groups.size(), Form{});
+    if (char const * env = std::getenv("SC_EULER_FORM"))
     {
-        char const c = env[0];
-        if (c == '0' || c == '1' || c == 'a' || c == '3')
+        auto parse = [](std::string const & t, int & code)
         {
-            for (size_t gi = 0; gi < groups.size(); ++gi)
+            if (t.empty() || t[0] < '1' || t[0] > '4' || t.size() > 2 || (t.size() == 2 && t[1] != 's'))
             {
-                int f = (c == '1' || (c == 'a' && ndim == 3)) ? 1 : 0;
-                if (c == '3' && form_exists(groups[gi].type, 2))
-                {
-                    f = 2;
-                }
-                gsplit[gi] = static_cast<char>(f);
+                return false;
+            }
+            code = (t[0] - '0') + (t.size() == 2 ? 10 : 0);
+            return true;
+        };
+        std::string const c(env);
+        Form want;
+        bool ok = false;
+        if (c.size() >= 2 && c[0] == 'f')
+        {
+            ok = parse(c.substr(1), want.vf);
+        }
+        else if (c.size() >= 4 && c[0] == 's' && c.find('/') != std::string::npos)
+        {
+            size_t const sl = c.find('/');
+            want.split = 1;
+            ok = parse(c.substr(1, sl - 1), want.v1) && parse(c.substr(sl + 1), want.v2);
+        }
+        if (ok)
+        {
+            bool const avail = want.split != 0 ? has_variant(1, want.v1) && has_variant(2, want.v2)
+                                               : has_variant(0, want.vf);
+            for (auto & f : gform)
+            {
+                f = avail ? want : Form{};
             }
             tuned = 1;
         }
@@ -2722,7 +3007,8 @@ int sceuler_counts(void * h, int64_t * out)
     return 0;
 }
 
-// which: 0 sigma0, 1 taumin, 2 tauscale, 3 time_increment
+// which: 0 sigma0, 1 taumin, 2 tauscale, 3 time_increment, 4 1 once the
+// kernel forms are tuned (else -1), 100 + t the form code of cell type t
 double sceuler_get_param(void * h, int which)
 {
     DEuler const & e = *static_cast<DEuler *>(h);
@@ -2732,8 +3018,13 @@ double sceuler_get_param(void * h, int which)
     case 0: return e.sigma0;
     case 1: return e.taumin;
     case 2: return e.tauscale;
-    case 4: return static_cast<double>(e.split_mask());
-    default: return e.dt;
+    case 4: return e.tuned != 0 ? 1.0 : -1.0;
+    default:
+        if (which >= 100 && which < 116)
+        {
+            return static_cast<double>(e.form_code(which - 100));
+        }
+        return e.dt;
     }
 }
 

@@ -14,6 +14,8 @@
 
 #include "mesh.hpp"
 
+#include <type_traits>
+
 namespace scamd
 {
 
@@ -123,147 +125,188 @@ struct Layout
     static constexpr int NFIELD = FD + 1;
 };
 
-// EulerJacobian<2>::update (euler_march.cpp:26-83).
-struct Jac2
+// Compile-time loop: f(std::integral_constant<int, 0>) ... f(<N - 1>).
+template <int I, int N, class F>
+__device__ inline void static_for(F && f)
 {
-    double j[4][4][2];
-    double f[4][2];
-
-    __device__ inline void update(double gamma, double const * sol)
+    if constexpr (I < N)
     {
-        double const ga = gamma;
-        double const ga1 = ga - 1;
-        double const ga3 = ga - 3;
-        double const ga1h = ga1 / 2;
-        double const u1 = sol[0] + TINYJ;
-        double const u2 = sol[1];
-        double const u3 = sol[2];
-        double const u4 = sol[3];
-        double const rho2 = u1 * u1;
-        double const v1 = u2 / u1;
-        double const v1o2 = v1 * v1;
-        double const v2 = u3 / u1;
-        double const v2o2 = v2 * v2;
-        double const ke2 = (u2 * u2 + u3 * u3) / u1;
-        double const g1ke2 = ga1 * ke2;
-        double const vs = ke2 / u1;
-        double const gretot = ga * u4;
-        double const getot = gretot / u1;
-        double const pr = ga1 * u4 - ga1h * ke2;
-        f[0][0] = u2;
-        f[0][1] = u3;
-        f[1][0] = pr + u2 * v1;
-        f[1][1] = u2 * v2;
-        f[2][0] = u3 * v1;
-        f[2][1] = pr + u3 * v2;
-        f[3][0] = (pr + u4) * v1;
-        f[3][1] = (pr + u4) * v2;
-        j[0][0][0] = 0; j[0][0][1] = 0;
-        j[0][1][0] = 1; j[0][1][1] = 0;
-        j[0][2][0] = 0; j[0][2][1] = 1;
-        j[0][3][0] = 0; j[0][3][1] = 0;
-        j[1][0][0] = -v1o2 + ga1h * vs; j[1][0][1] = -v1 * v2;
-        j[1][1][0] = -ga3 * v1; j[1][1][1] = v2;
-        j[1][2][0] = -ga1 * v2; j[1][2][1] = v1;
-        j[1][3][0] = ga1; j[1][3][1] = 0;
-        j[2][0][0] = -v2 * v1; j[2][0][1] = -v2o2 + ga1h * vs;
-        j[2][1][0] = v2; j[2][1][1] = -ga1 * v1;
-        j[2][2][0] = v1; j[2][2][1] = -ga3 * v2;
-        j[2][3][0] = 0; j[2][3][1] = ga1;
-        j[3][0][0] = (-gretot + g1ke2) * u2 / rho2; j[3][0][1] = (-gretot + g1ke2) * u3 / rho2;
-        j[3][1][0] = getot - ga1h * (vs + 2 * v1o2); j[3][1][1] = -ga1 * v1 * v2;
-        j[3][2][0] = -ga1 * v2 * v1; j[3][2][1] = getot - ga1h * (vs + 2 * v2o2);
-        j[3][3][0] = ga * v1; j[3][3][1] = ga * v2;
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
     }
-};
+}
 
-// EulerJacobian<3>::update (euler_march.cpp:85-158).
-struct Jac3
-{
-    double j[5][5][3];
-    double f[5][3];
-
-    __device__ inline void update(double gamma, double const * sol)
-    {
-        double const ga = gamma;
-        double const ga1 = ga - 1;
-        double const ga3 = ga - 3;
-        double const ga1h = ga1 / 2;
-        double const u1 = sol[0] + TINYJ;
-        double const u2 = sol[1];
-        double const u3 = sol[2];
-        double const u4 = sol[3];
-        double const u5 = sol[4];
-        double const rho2 = u1 * u1;
-        double const v1 = u2 / u1;
-        double const v1o2 = v1 * v1;
-        double const v2 = u3 / u1;
-        double const v2o2 = v2 * v2;
-        double const v3 = u4 / u1;
-        double const v3o2 = v3 * v3;
-        double const ke2 = (u2 * u2 + u3 * u3 + u4 * u4) / u1;
-        double const g1ke2 = ga1 * ke2;
-        double const vs = ke2 / u1;
-        double const gretot = ga * u5;
-        double const getot = gretot / u1;
-        double const pr = ga1 * u5 - ga1h * ke2;
-        f[0][0] = u2; f[0][1] = u3; f[0][2] = u4;
-        f[1][0] = pr + u2 * v1; f[1][1] = u2 * v2; f[1][2] = u2 * v3;
-        f[2][0] = u3 * v1; f[2][1] = pr + u3 * v2; f[2][2] = u3 * v3;
-        f[3][0] = u4 * v1; f[3][1] = u4 * v2; f[3][2] = pr + u4 * v3;
-        f[4][0] = (pr + u5) * v1; f[4][1] = (pr + u5) * v2; f[4][2] = (pr + u5) * v3;
-        j[0][0][0] = 0; j[0][0][1] = 0; j[0][0][2] = 0;
-        j[0][1][0] = 1; j[0][1][1] = 0; j[0][1][2] = 0;
-        j[0][2][0] = 0; j[0][2][1] = 1; j[0][2][2] = 0;
-        j[0][3][0] = 0; j[0][3][1] = 0; j[0][3][2] = 1;
-        j[0][4][0] = 0; j[0][4][1] = 0; j[0][4][2] = 0;
-        j[1][0][0] = -v1o2 + ga1h * vs; j[1][0][1] = -v1 * v2; j[1][0][2] = -v1 * v3;
-        j[1][1][0] = -ga3 * v1; j[1][1][1] = v2; j[1][1][2] = v3;
-        j[1][2][0] = -ga1 * v2; j[1][2][1] = v1; j[1][2][2] = 0;
-        j[1][3][0] = -ga1 * v3; j[1][3][1] = 0; j[1][3][2] = v1;
-        j[1][4][0] = ga1; j[1][4][1] = 0; j[1][4][2] = 0;
-        j[2][0][0] = -v2 * v1; j[2][0][1] = -v2o2 + ga1h * vs; j[2][0][2] = -v2 * v3;
-        j[2][1][0] = v2; j[2][1][1] = -ga1 * v1; j[2][1][2] = 0;
-        j[2][2][0] = v1; j[2][2][1] = -ga3 * v2; j[2][2][2] = v3;
-        j[2][3][0] = 0; j[2][3][1] = -ga1 * v3; j[2][3][2] = v2;
-        j[2][4][0] = 0; j[2][4][1] = ga1; j[2][4][2] = 0;
-        j[3][0][0] = -v3 * v1; j[3][0][1] = -v3 * v2; j[3][0][2] = -v3o2 + ga1h * vs;
-        j[3][1][0] = v3; j[3][1][1] = 0; j[3][1][2] = -ga1 * v1;
-        j[3][2][0] = 0; j[3][2][1] = v3; j[3][2][2] = -ga1 * v2;
-        j[3][3][0] = v1; j[3][3][1] = v2; j[3][3][2] = -ga3 * v3;
-        j[3][4][0] = 0; j[3][4][1] = 0; j[3][4][2] = ga1;
-        j[4][0][0] = (-gretot + g1ke2) * u2 / rho2;
-        j[4][0][1] = (-gretot + g1ke2) * u3 / rho2;
-        j[4][0][2] = (-gretot + g1ke2) * u4 / rho2;
-        j[4][1][0] = getot - ga1h * (vs + 2 * v1o2); j[4][1][1] = -ga1 * v1 * v2; j[4][1][2] = -ga1 * v1 * v3;
-        j[4][2][0] = -ga1 * v2 * v1; j[4][2][1] = getot - ga1h * (vs + 2 * v2o2); j[4][2][2] = -ga1 * v2 * v3;
-        j[4][3][0] = -ga1 * v3 * v1; j[4][3][1] = -ga1 * v3 * v2; j[4][3][2] = getot - ga1h * (vs + 2 * v3o2);
-        j[4][4][0] = ga * v1; j[4][4][1] = ga * v2; j[4][4][2] = ga * v3;
-    }
-};
-
+// The flux Jacobian of one state (EulerJacobian<ND>::update,
+// euler_march.cpp:26-158), evaluated one equation row at a time: set()
+// forms the shared scalars, row<K>() the K-th rows of the flux (f[d]) and
+// of the Jacobian (j[m][d]) with the reference's expressions.  A row is
+// only live while it is used, which keeps the register footprint of the
+// flux integral small.
 template <int ND>
-struct JacOf;
+struct JacS;
+
 template <>
-struct JacOf<2>
+struct JacS<2>
 {
-    using type = Jac2;
+    double ga, ga1, ga3, ga1h, u1, u2, u3, u4, rho2, v1, v1o2, v2, v2o2, ke2, g1ke2, vs, gretot, getot, pr;
+
+    __device__ inline void set(double gamma, double const * sol)
+    {
+        ga = gamma;
+        ga1 = ga - 1;
+        ga3 = ga - 3;
+        ga1h = ga1 / 2;
+        u1 = sol[0] + TINYJ;
+        u2 = sol[1];
+        u3 = sol[2];
+        u4 = sol[3];
+        rho2 = u1 * u1;
+        v1 = u2 / u1;
+        v1o2 = v1 * v1;
+        v2 = u3 / u1;
+        v2o2 = v2 * v2;
+        ke2 = (u2 * u2 + u3 * u3) / u1;
+        g1ke2 = ga1 * ke2;
+        vs = ke2 / u1;
+        gretot = ga * u4;
+        getot = gretot / u1;
+        pr = ga1 * u4 - ga1h * ke2;
+    }
+
+    template <int K>
+    __device__ inline void row(double (&j)[4][2], double (&f)[2]) const
+    {
+        if constexpr (K == 0)
+        {
+            f[0] = u2; f[1] = u3;
+            j[0][0] = 0; j[0][1] = 0;
+            j[1][0] = 1; j[1][1] = 0;
+            j[2][0] = 0; j[2][1] = 1;
+            j[3][0] = 0; j[3][1] = 0;
+        }
+        else if constexpr (K == 1)
+        {
+            f[0] = pr + u2 * v1; f[1] = u2 * v2;
+            j[0][0] = -v1o2 + ga1h * vs; j[0][1] = -v1 * v2;
+            j[1][0] = -ga3 * v1; j[1][1] = v2;
+            j[2][0] = -ga1 * v2; j[2][1] = v1;
+            j[3][0] = ga1; j[3][1] = 0;
+        }
+        else if constexpr (K == 2)
+        {
+            f[0] = u3 * v1; f[1] = pr + u3 * v2;
+            j[0][0] = -v2 * v1; j[0][1] = -v2o2 + ga1h * vs;
+            j[1][0] = v2; j[1][1] = -ga1 * v1;
+            j[2][0] = v1; j[2][1] = -ga3 * v2;
+            j[3][0] = 0; j[3][1] = ga1;
+        }
+        else
+        {
+            f[0] = (pr + u4) * v1; f[1] = (pr + u4) * v2;
+            j[0][0] = (-gretot + g1ke2) * u2 / rho2; j[0][1] = (-gretot + g1ke2) * u3 / rho2;
+            j[1][0] = getot - ga1h * (vs + 2 * v1o2); j[1][1] = -ga1 * v1 * v2;
+            j[2][0] = -ga1 * v2 * v1; j[2][1] = getot - ga1h * (vs + 2 * v2o2);
+            j[3][0] = ga * v1; j[3][1] = ga * v2;
+        }
+    }
 };
+
 template <>
-struct JacOf<3>
+struct JacS<3>
 {
-    using type = Jac3;
+    double ga, ga1, ga3, ga1h, u1, u2, u3, u4, u5, rho2, v1, v1o2, v2, v2o2, v3, v3o2, ke2, g1ke2, vs, gretot, getot,
+        pr;
+
+    __device__ inline void set(double gamma, double const * sol)
+    {
+        ga = gamma;
+        ga1 = ga - 1;
+        ga3 = ga - 3;
+        ga1h = ga1 / 2;
+        u1 = sol[0] + TINYJ;
+        u2 = sol[1];
+        u3 = sol[2];
+        u4 = sol[3];
+        u5 = sol[4];
+        rho2 = u1 * u1;
+        v1 = u2 / u1;
+        v1o2 = v1 * v1;
+        v2 = u3 / u1;
+        v2o2 = v2 * v2;
+        v3 = u4 / u1;
+        v3o2 = v3 * v3;
+        ke2 = (u2 * u2 + u3 * u3 + u4 * u4) / u1;
+        g1ke2 = ga1 * ke2;
+        vs = ke2 / u1;
+        gretot = ga * u5;
+        getot = gretot / u1;
+        pr = ga1 * u5 - ga1h * ke2;
+    }
+
+    template <int K>
+    __device__ inline void row(double (&j)[5][3], double (&f)[3]) const
+    {
+        if constexpr (K == 0)
+        {
+            f[0] = u2; f[1] = u3; f[2] = u4;
+            j[0][0] = 0; j[0][1] = 0; j[0][2] = 0;
+            j[1][0] = 1; j[1][1] = 0; j[1][2] = 0;
+            j[2][0] = 0; j[2][1] = 1; j[2][2] = 0;
+            j[3][0] = 0; j[3][1] = 0; j[3][2] = 1;
+            j[4][0] = 0; j[4][1] = 0; j[4][2] = 0;
+        }
+        else if constexpr (K == 1)
+        {
+            f[0] = pr + u2 * v1; f[1] = u2 * v2; f[2] = u2 * v3;
+            j[0][0] = -v1o2 + ga1h * vs; j[0][1] = -v1 * v2; j[0][2] = -v1 * v3;
+            j[1][0] = -ga3 * v1; j[1][1] = v2; j[1][2] = v3;
+            j[2][0] = -ga1 * v2; j[2][1] = v1; j[2][2] = 0;
+            j[3][0] = -ga1 * v3; j[3][1] = 0; j[3][2] = v1;
+            j[4][0] = ga1; j[4][1] = 0; j[4][2] = 0;
+        }
+        else if constexpr (K == 2)
+        {
+            f[0] = u3 * v1; f[1] = pr + u3 * v2; f[2] = u3 * v3;
+            j[0][0] = -v2 * v1; j[0][1] = -v2o2 + ga1h * vs; j[0][2] = -v2 * v3;
+            j[1][0] = v2; j[1][1] = -ga1 * v1; j[1][2] = 0;
+            j[2][0] = v1; j[2][1] = -ga3 * v2; j[2][2] = v3;
+            j[3][0] = 0; j[3][1] = -ga1 * v3; j[3][2] = v2;
+            j[4][0] = 0; j[4][1] = ga1; j[4][2] = 0;
+        }
+        else if constexpr (K == 3)
+        {
+            f[0] = u4 * v1; f[1] = u4 * v2; f[2] = pr + u4 * v3;
+            j[0][0] = -v3 * v1; j[0][1] = -v3 * v2; j[0][2] = -v3o2 + ga1h * vs;
+            j[1][0] = v3; j[1][1] = 0; j[1][2] = -ga1 * v1;
+            j[2][0] = 0; j[2][1] = v3; j[2][2] = -ga1 * v2;
+            j[3][0] = v1; j[3][1] = v2; j[3][2] = -ga3 * v3;
+            j[4][0] = 0; j[4][1] = 0; j[4][2] = ga1;
+        }
+        else
+        {
+            f[0] = (pr + u5) * v1; f[1] = (pr + u5) * v2; f[2] = (pr + u5) * v3;
+            j[0][0] = (-gretot + g1ke2) * u2 / rho2;
+            j[0][1] = (-gretot + g1ke2) * u3 / rho2;
+            j[0][2] = (-gretot + g1ke2) * u4 / rho2;
+            j[1][0] = getot - ga1h * (vs + 2 * v1o2); j[1][1] = -ga1 * v1 * v2; j[1][2] = -ga1 * v1 * v3;
+            j[2][0] = -ga1 * v2 * v1; j[2][1] = getot - ga1h * (vs + 2 * v2o2); j[2][2] = -ga1 * v2 * v3;
+            j[3][0] = -ga1 * v3 * v1; j[3][1] = -ga1 * v3 * v2; j[3][2] = getot - ga1h * (vs + 2 * v3o2);
+            j[4][0] = ga * v1; j[4][1] = ga * v2; j[4][2] = ga * v3;
+        }
+    }
 };
 
 // so0t of one row: -(J . so1c) summed over the dimension then the equation
 // (calc_solt_impl, euler_march.cpp:170-190).
-template <int ND, class J>
-__device__ inline void solt_of(J const & jac, double const (&s1)[ND + 2][ND], double (&st)[ND + 2])
+template <int ND>
+__device__ inline void solt_of(JacS<ND> const & jac, double const (&s1)[ND + 2][ND], double (&st)[ND + 2])
 {
     constexpr int NEQ = ND + 2;
-#pragma unroll
-    for (int ieq = 0; ieq < NEQ; ++ieq)
-    {
+    static_for<0, NEQ>([&](auto kc) {
+        constexpr int ieq = decltype(kc)::value;
+        double jr[NEQ][ND];
+        double fr[ND];
+        jac.template row<ieq>(jr, fr);
         double val = 0.0;
 #pragma unroll
         for (int idm = 0; idm < ND; ++idm)
@@ -271,11 +314,11 @@ __device__ inline void solt_of(J const & jac, double const (&s1)[ND + 2][ND], do
 #pragma unroll
             for (int jeq = 0; jeq < NEQ; ++jeq)
             {
-                val += jac.j[ieq][jeq][idm] * s1[jeq][idm];
+                val += jr[jeq][idm] * s1[jeq][idm];
             }
         }
         st[ieq] = -val;
-    }
+    });
 }
 
 // Row-major ND x ND helpers of the boundary conditions (euler_bc.cpp:36-81).

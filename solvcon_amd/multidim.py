@@ -176,19 +176,23 @@ class EulerCore(object):
 
     @property
     def kernel_forms(self):
-        """{cell type id: 'fused' | 'fused3' | 'split'} chosen by the
-        autotuner of the first substep (None before it; no reference
-        counterpart); 'fused3' is the fused kernel compiled for three
-        wavefronts per SIMD."""
-        m = int(L.lib().sceuler_get_param(self._h, 4))
-        if m < 0:
+        """{cell type id: form} chosen by the autotuner of the first substep
+        (None before it; no reference counterpart).  A form is 'fV', the
+        fused kernel in variant V, or 'sA/B', the split pair (soln + BC +
+        CFL, then dsoln) in variants A and B.  A variant is the minimum
+        wavefronts per SIMD it is compiled for, followed by 's' when the
+        block's own rows are staged in LDS."""
+        if L.lib().sceuler_get_param(self._h, 4) < 0:
             return None
+
+        def var(c):
+            return '%d%s' % (c % 10, 's' if c >= 10 else '')
         types = np.unique(self.mesh.cltpn.ndarray[self.mesh.ngstcell:])
         out = {}
         for t in types:
-            t = int(t)
-            out[t] = 'split' if (m >> t) & 1 else \
-                'fused3' if (m >> (16 + t)) & 1 else 'fused'
+            c = int(L.lib().sceuler_get_param(self._h, 100 + int(t)))
+            out[int(t)] = 'f' + var(c) if c < 10000 else 's%s/%s' % (
+                var((c - 10000) // 100), var(c % 100))
         return out
 
     sigma0 = _param(0)

@@ -220,14 +220,21 @@ def test_march_bitwise(orc, name, sfc):
     assert np.array_equal(held, oe.so0n)
 
 
-@pytest.mark.parametrize('form', ['0', '1', '3', 'a'])
-@pytest.mark.parametrize('name', ['tri16', 'channel', 'mixed3d'])
+# Every compiled kernel variant: fV the fused kernel in variant V, sA/B the
+# split pair; a variant is the minimum wavefronts per SIMD, with 's' when
+# the block's rows are staged in LDS.
+FORMS_2D = ('f1', 'f3', 'f1s', 's1/1')
+FORMS_3D = ('f1', 'f2', 's1/1', 's1s/1', 's1/2', 's1s/2s')
+
+
+@pytest.mark.parametrize('name,form',
+                         [(n, f) for n in ('tri16', 'channel') for f in FORMS_2D]
+                         + [('mixed3d', f) for f in FORMS_3D])
 def test_march_every_kernel_form(orc, monkeypatch, name, form):
-    # The autotuner may pick any form per cell type; each fixed form gives
-    # the oracle's values (0 fused, 1 split pair, 3 three-wavefront fused
-    # kernel in 2D, a split in 3D).
+    # The autotuner may pick any variant per cell type; each fixed variant
+    # gives the oracle's values.
     print('case: march', name, 'form', form, flush=True)
-    monkeypatch.setenv('SC_EULER_SPLIT', form)
+    monkeypatch.setenv('SC_EULER_FORM', form)
     defn = _defs()[name]
     g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
     _riemann(ge, oe, o, seed=4)
@@ -241,28 +248,21 @@ def test_march_every_kernel_form(orc, monkeypatch, name, form):
     ge.march(7)
     oe.march(7)
     pair.same_euler(ge, oe)
-    forms = ge.kernel_forms
-    want = {'0': 'fused', '1': 'split'}.get(form)
-    for t, f in forms.items():
-        if want is not None:
-            assert f == want
-        elif form == '3':
-            assert f == ('fused3' if t in (3, 4) else 'fused')
-        else:
-            assert f == ('split' if t >= 5 else 'fused')
+    assert set(ge.kernel_forms.values()) == {form}
 
 
 def test_autotuner_reports_forms(orc, monkeypatch):
     print('case: autotuner forms', flush=True)
-    monkeypatch.delenv('SC_EULER_SPLIT', raising=False)
+    monkeypatch.delenv('SC_EULER_FORM', raising=False)
     defn = _defs()['mixed3d']
     g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
     assert ge.kernel_forms is None
     ge.init_solution(GAMMA, 1.0, [0.1, 0.0, 0.0], 1.0)
     ge.march(1)
-    assert set(ge.kernel_forms) == set(int(t) for t in
-                                      np.unique(g.cltpn.ndarray[g.ngstcell:]))
-    assert set(ge.kernel_forms.values()) <= {'fused', 'split', 'fused3'}
+    forms = ge.kernel_forms
+    assert set(forms) == set(int(t) for t in
+                             np.unique(g.cltpn.ndarray[g.ngstcell:]))
+    assert set(forms.values()) <= set(FORMS_3D)
 
 
 def test_fused_equals_stages(orc):
