@@ -487,10 +487,14 @@ def main():
     ap.add_argument('--warmup', type=int, default=None)
     ap.add_argument('--config', choices=('2d', '3d', '1d'), default='2d')
     ap.add_argument('--size', type=int, default=1024)
+    ap.add_argument('--form', default=None,
+                    help='fix the multidim kernel forms instead of the '
+                         'autotuner (SC_EULER_FORM syntax, e.g. '
+                         '"6:f2,5:s1s/1")')
     ap.add_argument('--size3', type=int, default=200)
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--cpu-size3', type=int, default=60)
-    ap.add_argument('--cpu-steps3', type=int, default=4)
+    ap.add_argument('--cpu-steps3', type=int, default=40)
     ap.add_argument('--copy-probe', type=int, default=1024,
                     help='MiB moved by the stream-copy probe (0: skip)')
     ap.add_argument('--no-cpu', action='store_true')
@@ -510,6 +514,8 @@ def main():
               'supported (every GPU box here has one GPU)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.form:
+        os.environ['SC_EULER_FORM'] = args.form
     defaults = {'2d': (500, 20), '3d': (200, 5), '1d': (21, 3)}[args.config]
     if args.steps is None:
         args.steps = defaults[0]

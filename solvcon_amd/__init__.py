@@ -10,7 +10,9 @@ from . import onedim  # noqa: F401
 from . import spacetime  # noqa: F401
 from ._lib import lib as _load_lib  # noqa: F401
 from .mesh import StaticMesh, StaticMeshBc  # noqa: F401
-from .multidim import EulerCore, GradientElement, EulerField  # noqa: F401
+from .gmsh import Gmsh  # noqa: F401
+from .multidim import (EulerCore, GradientElement, EulerField,  # noqa: F401
+                       RunHistory)
 
-__all__ = ['onedim', 'spacetime', 'StaticMesh', 'StaticMeshBc', 'EulerCore',
-           'GradientElement', 'EulerField']
+__all__ = ['onedim', 'spacetime', 'StaticMesh', 'StaticMeshBc', 'Gmsh',
+           'EulerCore', 'GradientElement', 'EulerField', 'RunHistory']

@@ -2371,14 +2371,16 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         batch_set_dt(d.data());
     }
     // The form of every group is timed by the first substep unless
-    // SC_EULER_FORM fixes it: fV (the fused kernel in variant V) or sA/B
-    // (the split pair in variants A and B), where a variant is the minimum
-    // wavefronts per SIMD followed by 's' when LDS-staged (f1, f3, f1s,
-    // s1s/2); groups whose type lacks a variant run f1.
+    // SC_EULER_FORM fixes it: a comma-separated list of entries, each a
+    // form for every group or "T:form" for the group of cell type T.  A
+    // form is fV (the fused kernel in variant V) or sA/B (the split pair in
+    // variants A and B), where a variant is the minimum wavefronts per SIMD
+    // followed by 's' when LDS-staged (f1, f3, f1s, s1s/2).  A group whose
+    // type lacks the variant runs f1.
     gform.assign(groups.size(), Form{});
     if (char const * env = std::getenv("SC_EULER_FORM"))
     {
-        auto parse = [](std::string const & t, int & code)
+        auto parse_var = [](std::string const & t, int & code)
         {
             if (t.empty() || t[0] < '1' || t[0] > '4' || t.size() > 2 || (t.size() == 2 && t[1] != 's'))
             {
@@ -2387,27 +2389,53 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
             code = (t[0] - '0') + (t.size() == 2 ? 10 : 0);
             return true;
         };
-        std::string const c(env);
-        Form want;
-        bool ok = false;
-        if (c.size() >= 2 && c[0] == 'f')
+        auto parse_form = [&](std::string const & c, Form & want)
         {
-            ok = parse(c.substr(1), want.vf);
-        }
-        else if (c.size() >= 4 && c[0] == 's' && c.find('/') != std::string::npos)
-        {
+            if (c.size() >= 2 && c[0] == 'f')
+            {
+                return parse_var(c.substr(1), want.vf);
+            }
             size_t const sl = c.find('/');
-            want.split = 1;
-            ok = parse(c.substr(1, sl - 1), want.v1) && parse(c.substr(sl + 1), want.v2);
-        }
-        if (ok)
+            if (c.size() >= 4 && c[0] == 's' && sl != std::string::npos)
+            {
+                want.split = 1;
+                return parse_var(c.substr(1, sl - 1), want.v1) && parse_var(c.substr(sl + 1), want.v2);
+            }
+            return false;
+        };
+        std::string const all(env);
+        bool any = false;
+        size_t pos = 0;
+        while (pos <= all.size())
         {
+            size_t const end = std::min(all.find(',', pos), all.size());
+            std::string item = all.substr(pos, end - pos);
+            pos = end + 1;
+            int type = -1;
+            size_t const colon = item.find(':');
+            if (colon != std::string::npos)
+            {
+                type = std::atoi(item.substr(0, colon).c_str());
+                item = item.substr(colon + 1);
+            }
+            Form want;
+            if (!parse_form(item, want))
+            {
+                continue;
+            }
             bool const avail = want.split != 0 ? has_variant(1, want.v1) && has_variant(2, want.v2)
                                                : has_variant(0, want.vf);
-            for (auto & f : gform)
+            for (size_t gi = 0; gi < groups.size(); ++gi)
             {
-                f = avail ? want : Form{};
+                if (type < 0 || groups[gi].type == type)
+                {
+                    gform[gi] = avail ? want : Form{};
+                }
             }
+            any = true;
+        }
+        if (any)
+        {
             tuned = 1;
         }
     }

@@ -6,6 +6,7 @@ Drop-in for solvcon's ``EulerCore`` and ``GradientElement`` bindings
 geometry live in HBM; every stage runs a HIP kernel of euler.hip.
 """
 
+import collections
 import ctypes
 
 import numpy as np
@@ -13,7 +14,8 @@ import numpy as np
 from . import _lib as L
 from .array import GhostArray
 
-__all__ = ['EulerCore', 'GradientElement', 'EulerField']
+__all__ = ['EulerCore', 'GradientElement', 'EulerField', 'RunHistory',
+           'RunRecord']
 
 L.register({
     'sceuler_create': (L.c_int, [L.c_vp, L.c_dbl, L.c_int, L.P_vp]),
@@ -371,6 +373,44 @@ class EulerCore(object):
         L.check(L.lib().sceuler_reduce(self._h, L.dptr(out)))
         return float(out[0]), float(out[1]), float(out[2])
 
+    # -- snapshots (no reference counterpart) --------------------------------
+
+    #: Tables a snapshot holds: both solution levels, CFL numbers and gamma,
+    #: every row (ghosts first).
+    SNAPSHOT_TABLES = ('so0c', 'so0n', 'so1c', 'so1n', 'cflo', 'cflc',
+                       'gamma')
+
+    def snapshot(self):
+        """Host copy of the solver state that a later march reads (the
+        tables of SNAPSHOT_TABLES and time_increment).  Mesh, parameters and
+        boundary conditions are configuration and are not included."""
+        out = {t: getattr(self, t).ndarray.copy() for t in self.SNAPSHOT_TABLES}
+        out['time_increment'] = np.float64(self.time_increment)
+        return out
+
+    def restore(self, snap):
+        """Load a snapshot taken from a solver on an identical mesh; a march
+        from here reproduces the original run bit for bit."""
+        for t in self.SNAPSHOT_TABLES:
+            dst = getattr(self, t).ndarray
+            src = np.asarray(snap[t])
+            if src.shape != dst.shape or src.dtype != dst.dtype:
+                raise ValueError('EulerCore.restore: %s has shape %s (%s), '
+                                 'expected %s (%s)' % (t, src.shape, src.dtype,
+                                                       dst.shape, dst.dtype))
+        for t in self.SNAPSHOT_TABLES:
+            getattr(self, t).ndarray[...] = snap[t]
+        self.set_time_increment(float(snap['time_increment']))
+
+    def save(self, path):
+        """Write snapshot() to an .npz file."""
+        np.savez(path, **self.snapshot())
+
+    def load(self, path):
+        """restore() from an .npz file written by save()."""
+        with np.load(path, allow_pickle=False) as d:
+            self.restore({k: d[k] for k in d.files})
+
 
 for _name in ('cevol', 'cecnd', 'sfcnd', 'sfnml', 'so0c', 'so0n', 'so0t',
               'so1c', 'so1n', 'stm', 'cflo', 'cflc', 'gamma'):
@@ -556,3 +596,38 @@ class EulerField(object):
     def calc_cfl_range(self):
         cfl = self.cfl
         return float(cfl.min()), float(cfl.max())
+
+
+RunRecord = collections.namedtuple('RunRecord',
+                                   ['step', 'mass', 'cfl_min', 'cfl_max'])
+
+
+class RunHistory(object):
+    """The newest ``length`` (step, mass, cfl_min, cfl_max) records of a run
+    (the record kept by solvcon/pilot/apps/obsrefl/_session.py:39-70).
+    record() takes them from the solver's device reduction, so a long run
+    moves three numbers per record to the host."""
+
+    def __init__(self, length=2048):
+        self.records = collections.deque(maxlen=length)
+
+    def __len__(self):
+        return len(self.records)
+
+    def append(self, step, mass, cfl_min, cfl_max):
+        rec = RunRecord(step, mass, cfl_min, cfl_max)
+        self.records.append(rec)
+        return rec
+
+    def record(self, core, step):
+        """Append the mass and CFL range of an EulerCore at ``step``."""
+        mass, cmin, cmax = core.reduce()
+        return self.append(step, mass, cmin, cmax)
+
+    @property
+    def last(self):
+        return self.records[-1] if self.records else None
+
+    @property
+    def masses(self):
+        return [(r.step, r.mass) for r in self.records]

@@ -408,3 +408,67 @@ def test_euler_field():
     mass, lo, hi = svr.reduce()
     np.testing.assert_allclose(mass, f.calc_overall_mass(), rtol=1e-14)
     assert (lo, hi) == (0.25, 1.5)
+
+
+def test_snapshot_resume_bitwise(tmp_path):
+    # A run marched 3 + 4 steps equals one that saves after 3 steps and is
+    # resumed from the file by a fresh solver on the same mesh.
+    print('case: snapshot resume', flush=True)
+    import solvcon_amd as sc
+    defn = _defs()['mixed3d']
+    g = pair.gpu_mesh(defn)
+
+    def fresh():
+        e = sc.EulerCore(mesh=g, time_increment=2e-3)
+        e.init_solution(GAMMA, 1.0, [0.1, 0.0, 0.0], 1.0)
+        st = e.so0n.ndarray
+        st[e.ngstcell:, 0] += np.linspace(0.0, 0.4, e.ncell)
+        del st
+        faces = g.bndfcs.ndarray[:, 0].copy()
+        e.add_nonrefl(faces[0::2])
+        e.add_slipwall(faces[1::2])
+        e.bc_soln()
+        e.bc_dsoln()
+        return e
+    a = fresh()
+    a.march(3)
+    path = str(tmp_path / 'snap.npz')
+    a.save(path)
+    a.march(4)
+    b = fresh()
+    b.load(path)
+    b.march(4)
+    for t in ('so0n', 'so1n', 'cflo', 'cflc'):
+        assert np.array_equal(getattr(a, t).ndarray, getattr(b, t).ndarray), t
+    hist = sc.RunHistory(length=2)
+    for step in (7, 8, 9):
+        rec = hist.record(b, step)
+    assert len(hist) == 2 and hist.last == rec and hist.masses[0][0] == 8
+    mass, cmin, cmax = b.reduce()
+    assert (rec.mass, rec.cfl_min, rec.cfl_max) == (mass, cmin, cmax)
+    with pytest.raises(ValueError):
+        c = sc.EulerCore(mesh=pair.gpu_mesh(_defs()['hexblk']),
+                         time_increment=2e-3)
+        c.load(path)
+
+
+def test_per_type_forms(orc, monkeypatch):
+    # SC_EULER_FORM entries "T:form" fix the form of one cell type's group.
+    print('case: per-type forms', flush=True)
+    want = {5: 's1s/1', 6: 'f2', 7: 's1s/2s', 8: 's1/2'}
+    monkeypatch.setenv('SC_EULER_FORM',
+                       ','.join('%d:%s' % kv for kv in want.items()))
+    defn = _defs()['mixed3d']
+    g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
+    _riemann(ge, oe, o, seed=6)
+    faces = list(o.bndfcs[:, 0])
+    ge.add_nonrefl(faces)
+    oe.add_bc(orc.BC_NONREFL, faces)
+    ge.bc_soln()
+    ge.bc_dsoln()
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    ge.march(5)
+    oe.march(5)
+    pair.same_euler(ge, oe)
+    assert ge.kernel_forms == want
