@@ -133,9 +133,13 @@ void sceuler_destroy(void * h);
 int sceuler_counts(void * h, int64_t * out);
 /* sigma0 (0), taumin (1), tauscale (2), time_increment (3),
  * euler.hpp:95-102; the reference fixes time_increment at construction,
- * setting it here (not on a batched solver) has no counterpart.  Read-only
- * 4 (no counterpart): the autotuned kernel form, bit t set when the cells
- * of type t run the split kernel pair, -1 before the first substep. */
+ * setting it here (not on a batched solver) has no counterpart.  No
+ * counterpart either: 5, the cell count of a rank chunk of the batched march
+ * (0: all ranks at once); read-only 4, 1 once the first substep has tuned
+ * the kernel forms (else -1); read-only 100 + t, the form of the cells of
+ * type t: V for the fused kernel in variant V, 10000 + 100 A + B for the
+ * split pair in variants A and B (a variant is the minimum wavefronts per
+ * SIMD plus 10 when LDS-staged), 0 when the mesh has no such cells. */
 double sceuler_get_param(void * h, int which);
 int sceuler_set_param(void * h, int which, double v);
 /* info[4]: rows, elements per row, ghost rows, rank of the table. */

@@ -77,11 +77,12 @@ class BatchedEuler(object):
     [ncell_k, ndim] and the instance index; ``steps`` the step count of each
     instance; ``dts`` their time increments, or None for the dt that puts
     each instance's largest initial CFL number at ``cfl``.  Every instance
-    gets non-reflective boundaries and gamma.
+    gets non-reflective boundaries and gamma.  ``chunk_cells`` sets
+    ``EulerCore.batch_chunk``.
     """
 
     def __init__(self, defs, states, steps, dts=None, gamma=1.4, cfl=0.8,
-                 sfc=True):
+                 sfc=True, chunk_cells=0):
         self.ninst = len(defs)
         self.steps = np.asarray(steps, dtype=np.int64)
         if self.steps.size != self.ninst:
@@ -112,6 +113,7 @@ class BatchedEuler(object):
         else:
             self.dts = np.asarray(dts, dtype=np.float64)
             core.batch_set_dt(self.dts)
+        core.batch_chunk = chunk_cells
         self.core = core
         self.ncells = np.diff(self.coff)
 

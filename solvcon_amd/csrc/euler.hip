@@ -1850,18 +1850,19 @@ struct DEuler
 
     // One fused substep of the batched march over the cells of the first
     // nact ranks of every type group.
-    void run_cells_batch(int64_t nact)
+    void run_cells_batch(int64_t ra, int64_t rb)
     {
         KP const p = params();
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
             auto & gr = groups[gi];
-            int64_t const cnt = bcum[gi][nact];
-            if (cnt == 0)
+            int64_t const lo = bcum[gi][ra];
+            int64_t const cnt = bcum[gi][rb] - lo;
+            if (cnt <= 0)
             {
                 continue;
             }
-            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, cnt};
+            GV const g{gr.off + lo, gr.n, gr.nbr.ptr + lo, gr.geo.ptr + lo, cnt};
             unsigned const nb = grid_for(cnt, BLK);
             constexpr int MB = M_FUSED | M_BATCH;
             switch (gr.type)
@@ -1882,23 +1883,61 @@ struct DEuler
     // step over the ranks still marching, then calc_solt once.  A finished
     // instance keeps its state in place: the buffer swap of the others
     // returns to the same parity after each whole step.
-    void march_batch()
+    //
+    // The ranks are marched in chunks of consecutive ranks holding about
+    // chunk_cells cells (0: one chunk of every rank).  A chunk runs all its
+    // steps before the next starts, so its working set can stay in the
+    // last-level cache from step to step.  Every chunk swaps the buffers an
+    // even number of times, so each instance still ends in the same buffer,
+    // and each instance's arithmetic is unchanged.
+    int64_t chunk_cells = 0;
+
+    int64_t rank_cells(int64_t ra, int64_t rb) const
     {
-        int64_t const maxs = ninst > 0 ? bsteps[0] : 0;
-        int64_t nact = ninst;
+        int64_t c = 0;
+        for (auto const & cum : bcum)
+        {
+            c += cum[rb] - cum[ra];
+        }
+        return c;
+    }
+
+    void march_ranks(int64_t ra, int64_t rb)
+    {
+        int64_t const maxs = bsteps[ra];
+        int64_t nact = rb;
         for (int64_t step = 0; step < maxs; ++step)
         {
-            while (nact > 0 && bsteps[nact - 1] <= step)
+            while (nact > ra && bsteps[nact - 1] <= step)
             {
                 --nact;
             }
             for (int sub = 0; sub < 2; ++sub)
             {
                 update();
-                run_cells_batch(nact);
+                run_cells_batch(ra, nact);
             }
         }
-        if (maxs > 0)
+    }
+
+    void march_batch()
+    {
+        int64_t ra = 0;
+        while (ra < ninst)
+        {
+            int64_t rb = ninst;
+            if (chunk_cells > 0)
+            {
+                rb = ra + 1;
+                while (rb < ninst && rank_cells(ra, rb + 1) <= chunk_cells)
+                {
+                    ++rb;
+                }
+            }
+            march_ranks(ra, rb);
+            ra = rb;
+        }
+        if (ninst > 0 && bsteps[0] > 0)
         {
             calc_solt();
         }
@@ -3036,7 +3075,8 @@ int sceuler_counts(void * h, int64_t * out)
 }
 
 // which: 0 sigma0, 1 taumin, 2 tauscale, 3 time_increment, 4 1 once the
-// kernel forms are tuned (else -1), 100 + t the form code of cell type t
+// kernel forms are tuned (else -1), 5 the batch chunk size in cells, 100 + t
+// the form code of cell type t
 double sceuler_get_param(void * h, int which)
 {
     DEuler const & e = *static_cast<DEuler *>(h);
@@ -3047,6 +3087,7 @@ double sceuler_get_param(void * h, int which)
     case 1: return e.taumin;
     case 2: return e.tauscale;
     case 4: return e.tuned != 0 ? 1.0 : -1.0;
+    case 5: return static_cast<double>(e.chunk_cells);
     default:
         if (which >= 100 && which < 116)
         {
@@ -3074,6 +3115,13 @@ int sceuler_set_param(void * h, int which, double v)
                     scamd::fail(scamd::SC_ERUNTIME, "EulerCore: a batched solver sets dt per instance");
                 }
                 e.dt = v;
+                break;
+            case 5:
+                if (v < 0.0)
+                {
+                    scamd::fail(scamd::SC_EVALUE, "EulerCore: the batch chunk size must be >= 0 cells");
+                }
+                e.chunk_cells = static_cast<int64_t>(v);
                 break;
             default: scamd::fail(scamd::SC_EVALUE, "EulerCore: read-only or unknown parameter");
             }

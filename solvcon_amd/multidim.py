@@ -315,6 +315,19 @@ class EulerCore(object):
             self._after()
         return out
 
+    @property
+    def batch_chunk(self):
+        """Cells per rank chunk of the batched march: consecutive ranks
+        (instances by descending step count) holding about this many cells
+        run all their steps before the next chunk, so a chunk can stay in the
+        last-level cache; 0 marches every rank at once.  Results do not
+        depend on it."""
+        return int(L.lib().sceuler_get_param(self._h, 5))
+
+    @batch_chunk.setter
+    def batch_chunk(self, cells):
+        L.check(L.lib().sceuler_set_param(self._h, 5, float(cells)))
+
     def march_batch(self):
         """March every instance its own step count; returns the device
         time of the march in milliseconds."""
@@ -400,7 +413,9 @@ class EulerCore(object):
                                                        dst.shape, dst.dtype))
         for t in self.SNAPSHOT_TABLES:
             getattr(self, t).ndarray[...] = snap[t]
-        self.set_time_increment(float(snap['time_increment']))
+        if self._ninst == 0:
+            # A batched solver keeps its per-instance dt.
+            self.set_time_increment(float(snap['time_increment']))
 
     def save(self, path):
         """Write snapshot() to an .npz file."""

@@ -57,9 +57,10 @@ def _alone(defn, state, dt, steps, gamma=1.4):
     return mh, ec
 
 
-def _check_batch(defs, states, steps):
+def _check_batch(defs, states, steps, chunk=0):
     from solvcon_amd.batch import BatchedEuler
-    b = BatchedEuler(defs, states, steps)
+    b = BatchedEuler(defs, states, steps, chunk_cells=chunk)
+    assert b.core.batch_chunk == chunk
     b.march()
     gofs = 0
     for k, d in enumerate(defs):
@@ -81,8 +82,10 @@ def _check_batch(defs, states, steps):
 
 
 @pytest.mark.gpu
-def test_gpu_batch_equals_alone_2d():
-    print('case: batch of 6 2D instances', flush=True)
+@pytest.mark.parametrize('chunk', [0, 150])
+def test_gpu_batch_equals_alone_2d(chunk):
+    # chunk 150: rank chunks of about 150 cells, marched one after another.
+    print('case: batch of 6 2D instances, chunk', chunk, flush=True)
     from solvcon_amd.gen import riemann_box, quadrant_state
     from solvcon_amd.batch import build_mesh
     defs = _defs2d()
@@ -92,12 +95,13 @@ def test_gpu_batch_equals_alone_2d():
         cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
         rho, p = riemann_box(30 + k)
         states.append(quadrant_state(cen / cen.max(0), rho, p))
-    _check_batch(defs, states, [4, 7, 2, 7, 5, 1])
+    _check_batch(defs, states, [4, 7, 2, 7, 5, 1], chunk)
 
 
 @pytest.mark.gpu
-def test_gpu_batch_equals_alone_3d():
-    print('case: batch of 3 3D mixed instances', flush=True)
+@pytest.mark.parametrize('chunk', [0, 200])
+def test_gpu_batch_equals_alone_3d(chunk):
+    print('case: batch of 3 3D mixed instances, chunk', chunk, flush=True)
     from solvcon_amd.gen import mixed_mesh_def, blast_state
     from solvcon_amd.batch import build_mesh
     defs = [mixed_mesh_def(3, seed=3), mixed_mesh_def(4, seed=5),
@@ -107,7 +111,7 @@ def test_gpu_batch_equals_alone_3d():
         mh = build_mesh(d)
         cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
         states.append(blast_state(cen, r=0.4, center=cen.mean(0)))
-    _check_batch(defs, states, [2, 3, 1])
+    _check_batch(defs, states, [2, 3, 1], chunk)
 
 
 @pytest.mark.gpu
