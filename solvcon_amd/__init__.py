@@ -7,6 +7,7 @@ libsolvcon_amd.so on the GPU.
 """
 
 from . import onedim  # noqa: F401
+from . import obsrefl  # noqa: F401
 from . import spacetime  # noqa: F401
 from ._lib import lib as _load_lib  # noqa: F401
 from .mesh import StaticMesh, StaticMeshBc  # noqa: F401

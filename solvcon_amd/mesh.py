@@ -37,6 +37,15 @@ TABLES = ('ndcrd', 'fccnd', 'fcnml', 'fcara', 'clcnd', 'clvol', 'fctpn',
           'bndfcs')
 
 
+class _Table(np.ndarray):
+    """A host table indexed like a numpy array that also offers the
+    ``.ndarray`` view of the reference's SimpleArray."""
+
+    @property
+    def ndarray(self):
+        return self.view(np.ndarray)
+
+
 class StaticMeshBc(object):
     """One boundary-condition group: a name and its faces.  facn columns
     are (face index, row in bndfcs, related-block face = -1)
@@ -46,9 +55,10 @@ class StaticMeshBc(object):
 
     def __init__(self, name, facn0, facn1):
         self.name = name
-        self.facn = np.full((len(facn0), 3), -1, dtype='int32')
-        self.facn[:, 0] = facn0
-        self.facn[:, 1] = facn1
+        facn = np.full((len(facn0), 3), -1, dtype='int32')
+        facn[:, 0] = facn0
+        facn[:, 1] = facn1
+        self.facn = facn.view(_Table)
 
     @property
     def nbound(self):
