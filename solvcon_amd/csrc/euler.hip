@@ -118,17 +118,24 @@ constexpr int role_variant(int nd, int role, int v)
     {
         return role == 0 ? (v == 0 ? 1 : v == 1 ? 3 : 11) : (v == 0 ? 1 : 0);
     }
-    return role == 0 ? (v == 0 ? 1 : v == 1 ? 2 : 0) : role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : 12);
+    return role == 0 ? (v == 0 ? 1 : v == 1 ? 2 : 102) : role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : 12);
 }
 
+// Mode bits of a variant code: waves + 10 when LDS-staged + 100 when the
+// fused kernel reads the neighbours' so0t from memory (calc_solt first)
+// instead of recomputing it.
 constexpr int variant_mode(int code)
 {
-    return mode_waves(code % 10) | (code >= 10 ? M_STAGE : 0);
+    return mode_waves(code % 10) | ((code / 10) % 10 != 0 ? M_STAGE : 0) | (code >= 100 ? M_STMEM : 0);
 }
 
 constexpr bool variant_built(int nd, int mode)
 {
-    if ((mode & (M_WAVES | M_STAGE)) == 0)
+    bool const soln = (mode & M_SOLN) != 0;
+    bool const dsoln = (mode & M_DSOLN) != 0;
+    int const role = soln && dsoln ? 0 : soln ? 1 : 2;
+    int const vmask = M_WAVES | M_STAGE | (role == 0 ? M_STMEM : 0);
+    if ((mode & vmask) == 0)
     {
         return true;
     }
@@ -136,13 +143,10 @@ constexpr bool variant_built(int nd, int mode)
     {
         return false;
     }
-    bool const soln = (mode & M_SOLN) != 0;
-    bool const dsoln = (mode & M_DSOLN) != 0;
-    int const role = soln && dsoln ? 0 : soln ? 1 : 2;
     for (int v = 0; v < NVAR; ++v)
     {
         int const c = role_variant(nd, role, v);
-        if (c != 0 && variant_mode(c) == (mode & (M_WAVES | M_STAGE)))
+        if (c != 0 && (variant_mode(c) & vmask) == (mode & vmask))
         {
             return true;
         }
@@ -1834,6 +1838,7 @@ struct DEuler
         case 3: run_group<MODE | variant_mode(3)>(p, gr, cnt); break;
         case 11: run_group<MODE | variant_mode(11)>(p, gr, cnt); break;
         case 12: run_group<MODE | variant_mode(12)>(p, gr, cnt); break;
+        case 102: run_group<MODE | variant_mode(102)>(p, gr, cnt); break;
         default: run_group<MODE>(p, gr, cnt); break;
         }
     }
@@ -2023,7 +2028,7 @@ struct DEuler
         bool any = false;
         for (auto const & f : gform)
         {
-            any = any || f.split != 0;
+            any = any || needs_solt(f);
         }
         if (any)
         {
@@ -2034,6 +2039,9 @@ struct DEuler
             run_form(p, groups[gi], gform[gi]);
         }
     }
+
+    // Whether a form reads so0t from memory, so calc_solt must run first.
+    static bool needs_solt(Form const & f) { return f.split != 0 || f.vf >= 100; }
 
     void run_role(KP const & p, Group const & gr, int role, int code, bool warm = false)
     {
@@ -2152,15 +2160,20 @@ struct DEuler
                         int const c = role_variant(ndim, role, v);
                         if (t >= 0.0f)
                         {
-                            std::fprintf(stderr, "  w%d%s %.4f ms", c % 10, c >= 10 ? "+lds" : "", t);
+                            std::fprintf(stderr, "  w%d%s%s %.4f ms", c % 10, (c / 10) % 10 != 0 ? "+lds" : "",
+                                         c >= 100 ? "+so0t" : "", t);
                         }
                     }
                     std::fprintf(stderr, "\n");
                 }
             }
         }
-        std::vector<Form> fused(ng), split(ng);
-        std::vector<double> tf(ng, 0.0), ts(ng, 0.0);
+        // Per group: the best fused variant that recomputes so0t (fast),
+        // the best one that reads it (fastt), the best split pair.  Either
+        // no group reads so0t, or calc_solt runs once and every group takes
+        // its best form of all.
+        std::vector<Form> fast(ng), fastt(ng), split(ng);
+        std::vector<double> tf(ng, -1.0), tft(ng, -1.0), ts(ng, 0.0);
         for (size_t gi = 0; gi < ng; ++gi)
         {
             double best[3] = {-1.0, -1.0, -1.0};
@@ -2170,30 +2183,55 @@ struct DEuler
                 for (int v = 0; v < NVAR; ++v)
                 {
                     float const t = tm[(gi * 3 + role) * NVAR + v];
-                    if (t >= 0.0f && (best[role] < 0.0 || t < best[role]))
+                    int const c = role_variant(ndim, role, v);
+                    if (t < 0.0f)
+                    {
+                        continue;
+                    }
+                    if (role == 0)
+                    {
+                        double & b = c >= 100 ? tft[gi] : tf[gi];
+                        if (b < 0.0 || t < b)
+                        {
+                            b = t;
+                            (c >= 100 ? fastt[gi] : fast[gi]).vf = c;
+                        }
+                    }
+                    else if (best[role] < 0.0 || t < best[role])
                     {
                         best[role] = t;
-                        bc[role] = role_variant(ndim, role, v);
+                        bc[role] = c;
                     }
                 }
             }
-            fused[gi].vf = bc[0];
             split[gi].split = 1;
             split[gi].v1 = bc[1];
             split[gi].v2 = bc[2];
-            tf[gi] = best[0];
             ts[gi] = best[1] + best[2];
         }
-        double all_fused = 0.0;
-        double mixed = solt;
+        double no_solt = 0.0;
+        double with_solt = solt;
+        std::vector<Form> pick(ng);
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            all_fused += tf[gi];
-            mixed += std::min(tf[gi], ts[gi]);
+            no_solt += tf[gi];
+            double b = tf[gi];
+            pick[gi] = fast[gi];
+            if (tft[gi] >= 0.0 && tft[gi] < b)
+            {
+                b = tft[gi];
+                pick[gi] = fastt[gi];
+            }
+            if (ts[gi] < b)
+            {
+                b = ts[gi];
+                pick[gi] = split[gi];
+            }
+            with_solt += b;
         }
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            gform[gi] = (mixed < all_fused && ts[gi] < tf[gi]) ? split[gi] : fused[gi];
+            gform[gi] = with_solt < no_solt ? pick[gi] : fast[gi];
         }
         tuned = 1;
     }
@@ -2414,19 +2452,31 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
     // form for every group or "T:form" for the group of cell type T.  A
     // form is fV (the fused kernel in variant V) or sA/B (the split pair in
     // variants A and B), where a variant is the minimum wavefronts per SIMD
-    // followed by 's' when LDS-staged (f1, f3, f1s, s1s/2).  A group whose
-    // type lacks the variant runs f1.
+    // followed by 's' when LDS-staged and 't' when the fused kernel reads
+    // so0t from memory (f1, f3, f1s, f2t, s1s/2).  A group whose type lacks
+    // the variant runs f1.
     gform.assign(groups.size(), Form{});
     if (char const * env = std::getenv("SC_EULER_FORM"))
     {
         auto parse_var = [](std::string const & t, int & code)
         {
-            if (t.empty() || t[0] < '1' || t[0] > '4' || t.size() > 2 || (t.size() == 2 && t[1] != 's'))
+            if (t.empty() || t[0] < '1' || t[0] > '4')
             {
                 return false;
             }
-            code = (t[0] - '0') + (t.size() == 2 ? 10 : 0);
-            return true;
+            code = t[0] - '0';
+            size_t k = 1;
+            if (k < t.size() && t[k] == 's')
+            {
+                code += 10;
+                ++k;
+            }
+            if (k < t.size() && t[k] == 't')
+            {
+                code += 100;
+                ++k;
+            }
+            return k == t.size();
         };
         auto parse_form = [&](std::string const & c, Form & want)
         {
