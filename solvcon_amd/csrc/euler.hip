@@ -378,97 +378,50 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 solt_of<ND>(jac, j1, st[f]);
             }
             constexpr int NS = TR::nsub(f);
-            if constexpr (ND == 2)
+            // The solution at every sub-face centroid of this face, then
+            // the flux through each sub-face one Jacobian row (equation)
+            // at a time, so only one row is live; acc[k] still receives
+            // its sub-face terms in sub-face order.
+            double usfc[NS][NEQ];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
             {
-                // The whole Jacobian stays in registers; sub-face by
-                // sub-face, every equation.
-                double jf[NEQ][NEQ][ND];
-                double ff[NEQ][ND];
-                static_for<0, NEQ>([&](auto kc) {
-                    constexpr int k = decltype(kc)::value;
-                    jac.template row<k>(jf[k], ff[k]);
-                });
+                int const sf = TR::soff(f) + s;
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    usfc[s][k] = qdt * st[f][k];
+#pragma unroll
+                    for (int d = 0; d < ND; ++d)
+                    {
+                        usfc[s][k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
+                    }
+                }
+            }
+            static_for<0, NEQ>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                double jr[NEQ][ND];
+                double fr[ND];
+                jac.template row<k>(jr, fr);
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
                 {
                     int const sf = TR::soff(f) + s;
-                    double usfc[NEQ];
+                    double dot = 0.0;
 #pragma unroll
-                    for (int k = 0; k < NEQ; ++k)
+                    for (int d2 = 0; d2 < ND; ++d2)
                     {
-                        usfc[k] = qdt * st[f][k];
+                        double dfcn = fr[d2];
 #pragma unroll
-                        for (int d = 0; d < ND; ++d)
+                        for (int m = 0; m < NEQ; ++m)
                         {
-                            usfc[k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
+                            dfcn += jr[m][d2] * usfc[s][m];
                         }
+                        dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
                     }
-#pragma unroll
-                    for (int k = 0; k < NEQ; ++k)
-                    {
-                        double dot = 0.0;
-#pragma unroll
-                        for (int d2 = 0; d2 < ND; ++d2)
-                        {
-                            double dfcn = ff[k][d2];
-#pragma unroll
-                            for (int m = 0; m < NEQ; ++m)
-                            {
-                                dfcn += jf[k][m][d2] * usfc[m];
-                            }
-                            dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
-                        }
-                        acc[k] -= hdt * dot;
-                    }
+                    acc[k] -= hdt * dot;
                 }
-            }
-            else
-            {
-                // The solution at every sub-face centroid of this face, then
-                // the flux through each sub-face one Jacobian row (equation)
-                // at a time, so only one row is live; acc[k] still receives
-                // its sub-face terms in sub-face order.
-                double usfc[NS][NEQ];
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                {
-                    int const sf = TR::soff(f) + s;
-#pragma unroll
-                    for (int k = 0; k < NEQ; ++k)
-                    {
-                        usfc[s][k] = qdt * st[f][k];
-#pragma unroll
-                        for (int d = 0; d < ND; ++d)
-                        {
-                            usfc[s][k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
-                        }
-                    }
-                }
-                static_for<0, NEQ>([&](auto kc) {
-                    constexpr int k = decltype(kc)::value;
-                    double jr[NEQ][ND];
-                    double fr[ND];
-                    jac.template row<k>(jr, fr);
-#pragma unroll
-                    for (int s = 0; s < NS; ++s)
-                    {
-                        int const sf = TR::soff(f) + s;
-                        double dot = 0.0;
-#pragma unroll
-                        for (int d2 = 0; d2 < ND; ++d2)
-                        {
-                            double dfcn = fr[d2];
-#pragma unroll
-                            for (int m = 0; m < NEQ; ++m)
-                            {
-                                dfcn += jr[m][d2] * usfc[s][m];
-                            }
-                            dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
-                        }
-                        acc[k] -= hdt * dot;
-                    }
-                });
-            }
+            });
         });
         double const cvol = geo[LY::FV * n];
 #pragma unroll
