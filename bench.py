@@ -499,6 +499,9 @@ def main():
     ap.add_argument('--copy-probe', type=int, default=1024,
                     help='MiB moved by the stream-copy probe (0: skip)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--graph', action='store_true',
+                    help='replay a captured hipGraph per step instead of '
+                         'launching its kernels one by one')
     ap.add_argument('--batched', action='store_true',
                     help='measure the batched form instead')
     ap.add_argument('--batch-n', type=int, default=4096)
@@ -517,6 +520,8 @@ def main():
         sys.exit(2)
     if args.form:
         os.environ['SC_EULER_FORM'] = args.form
+    if args.graph:
+        os.environ['SC_EULER_GRAPH'] = '1'
     defaults = {'2d': (500, 20), '3d': (200, 5), '1d': (21, 3)}[args.config]
     if args.steps is None:
         args.steps = defaults[0]

@@ -135,11 +135,14 @@ int sceuler_counts(void * h, int64_t * out);
  * euler.hpp:95-102; the reference fixes time_increment at construction,
  * setting it here (not on a batched solver) has no counterpart.  No
  * counterpart either: 5, the cell count of a rank chunk of the batched march
- * (0: all ranks at once); read-only 4, 1 once the first substep has tuned
- * the kernel forms (else -1); read-only 100 + t, the form of the cells of
- * type t: V for the fused kernel in variant V, 10000 + 100 A + B for the
- * split pair in variants A and B (a variant is the minimum wavefronts per
- * SIMD plus 10 when LDS-staged), 0 when the mesh has no such cells. */
+ * (0: all ranks at once); 6, 1 when march and march_batch replay each whole
+ * step from a captured hipGraph, 0 (default) to launch its kernels one by
+ * one; read-only 4, 1 once the first substep has tuned the kernel forms
+ * (else -1); read-only 100 + t, the form of the cells of type t: V for the
+ * fused kernel in variant V (+ 100 when it reads so0t), 10000 + 100 A + B
+ * for the split pair in variants A and B (a variant is the minimum
+ * wavefronts per SIMD plus 10 when LDS-staged), 0 when the mesh has no such
+ * cells. */
 double sceuler_get_param(void * h, int which);
 int sceuler_set_param(void * h, int which, double v);
 /* info[4]: rows, elements per row, ghost rows, rank of the table. */

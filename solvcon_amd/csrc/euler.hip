@@ -1678,6 +1678,17 @@ struct DEuler
     StreamHolder sh;            // declared before the buffers: freed last
     hipStream_t stream = sh.s;
 
+    // With use_graph, whole steps (two substeps, the half-step pair) are
+    // enqueued once under stream capture and replayed as a hipGraph.  The
+    // kernels take every parameter by value, so a graph is replayed only
+    // while its key (the launch parameters, the forms and the active ranks)
+    // is unchanged.  Off by default: on this stack a replayed step costs
+    // more than launching its kernels directly.
+    int use_graph = 0;
+    hipGraphExec_t pgraph[2] = {nullptr, nullptr}; // plain march, by buffer parity
+    std::vector<unsigned char> pgraph_key[2];
+    std::vector<hipGraphExec_t> bgraphs; // batched march, one per active rank range
+
     struct Group
     {
         int32_t type = 0;
@@ -1708,9 +1719,62 @@ struct DEuler
 
     DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch = 0, int32_t const * inst_of_cell = nullptr,
            int64_t const * steps = nullptr);
-    ~DEuler() = default;
+    ~DEuler()
+    {
+        (void)hipStreamSynchronize(stream);
+        drop_graphs();
+    }
 
     void sync() { SC_HIP(hipStreamSynchronize(stream)); }
+
+    // Destroys the graph executables; their launches must have completed.
+    void drop_graphs()
+    {
+        for (int c = 0; c < 2; ++c)
+        {
+            if (pgraph[c] != nullptr)
+            {
+                (void)hipGraphExecDestroy(pgraph[c]);
+                pgraph[c] = nullptr;
+            }
+            pgraph_key[c].clear();
+        }
+        for (auto x : bgraphs)
+        {
+            (void)hipGraphExecDestroy(x);
+        }
+        bgraphs.clear();
+    }
+
+    // Captures what enqueue() puts on the stream into a graph executable.
+    // The buffer parity is restored if the capture fails.
+    template <class F>
+    hipGraphExec_t capture(F && enqueue)
+    {
+        int const c0 = cur;
+        hipGraph_t g = nullptr;
+        SC_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            enqueue();
+        }
+        catch (...)
+        {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g != nullptr)
+            {
+                (void)hipGraphDestroy(g);
+            }
+            cur = c0;
+            throw;
+        }
+        SC_HIP(hipStreamEndCapture(stream, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t const rc = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        SC_HIP(rc);
+        return ex;
+    }
 
     KP params(bool swap_after = false) const
     {
@@ -1860,15 +1924,40 @@ struct DEuler
         return c;
     }
 
+    // With use_graph, the step's launches are captured once per range of
+    // active ranks and the graph is replayed for every step of that range:
+    // a step swaps the buffers twice, so each replay starts from the same
+    // buffer parity as the capture.
     void march_ranks(int64_t ra, int64_t rb)
     {
         int64_t const maxs = bsteps[ra];
         int64_t nact = rb;
+        hipGraphExec_t g = nullptr;
+        int64_t gact = -1;
         for (int64_t step = 0; step < maxs; ++step)
         {
             while (nact > ra && bsteps[nact - 1] <= step)
             {
                 --nact;
+            }
+            if (use_graph != 0)
+            {
+                if (g == nullptr || gact != nact)
+                {
+                    g = capture(
+                        [&]()
+                        {
+                            for (int sub = 0; sub < 2; ++sub)
+                            {
+                                update();
+                                run_cells_batch(ra, nact);
+                            }
+                        });
+                    bgraphs.push_back(g);
+                    gact = nact;
+                }
+                SC_HIP(hipGraphLaunch(g, stream));
+                continue;
             }
             for (int sub = 0; sub < 2; ++sub)
             {
@@ -1894,6 +1983,15 @@ struct DEuler
             }
             march_ranks(ra, rb);
             ra = rb;
+        }
+        if (!bgraphs.empty())
+        {
+            sync();
+            for (auto x : bgraphs)
+            {
+                (void)hipGraphExecDestroy(x);
+            }
+            bgraphs.clear();
         }
         if (ninst > 0 && bsteps[0] > 0)
         {
@@ -2221,11 +2319,62 @@ struct DEuler
         return false;
     }
 
+    // What the launches of a step depend on: the kernel parameters at its
+    // start (buffer parity included) and every group's form.
+    std::vector<unsigned char> step_key() const
+    {
+        KP const p = params();
+        std::vector<unsigned char> k(sizeof(KP));
+        std::memcpy(k.data(), &p, sizeof(KP));
+        for (size_t gi = 0; gi < groups.size(); ++gi)
+        {
+            Group const & gr = groups[gi];
+            GV const g{gr.off, gr.n, gr.nbr.ptr, gr.geo.ptr, gr.n};
+            unsigned char const * a = reinterpret_cast<unsigned char const *>(&g);
+            unsigned char const * f = reinterpret_cast<unsigned char const *>(&gform[gi]);
+            k.insert(k.end(), a, a + sizeof(GV));
+            k.insert(k.end(), f, f + sizeof(Form));
+        }
+        return k;
+    }
+
+    // march (euler_march.cpp:350-359): after the tuning substep, whole
+    // steps are replayed from a captured graph when use_graph is set.
     void march(int64_t steps)
     {
-        for (int64_t it = 0; it < steps; ++it)
+        int64_t sub = 2 * steps;
+        if (sub > 0 && tuned == 0)
         {
             substep_fused();
+            --sub;
+        }
+        if (use_graph != 0 && sub >= 2)
+        {
+            std::vector<unsigned char> const key = step_key();
+            hipGraphExec_t & g = pgraph[cur];
+            if (g == nullptr || key != pgraph_key[cur])
+            {
+                if (g != nullptr)
+                {
+                    sync(); // launches of the old graph may still be queued
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+                g = capture(
+                    [&]()
+                    {
+                        substep_fused();
+                        substep_fused();
+                    });
+                pgraph_key[cur] = key;
+            }
+            for (; sub >= 2; sub -= 2)
+            {
+                SC_HIP(hipGraphLaunch(g, stream));
+            }
+        }
+        for (; sub > 0; --sub)
+        {
             substep_fused();
         }
         if (steps > 0)
@@ -2399,6 +2548,12 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         bqdt.alloc(ninst);
         std::vector<double> d(static_cast<size_t>(ninst), time_increment);
         batch_set_dt(d.data());
+    }
+    // SC_EULER_GRAPH=1 makes replaying captured graphs of whole steps the
+    // default instead of launching their kernels one by one.
+    if (char const * env = std::getenv("SC_EULER_GRAPH"))
+    {
+        use_graph = std::atoi(env) != 0 ? 1 : 0;
     }
     // The form of every group is timed by the first substep unless
     // SC_EULER_FORM fixes it: a comma-separated list of entries, each a
@@ -3078,8 +3233,9 @@ int sceuler_counts(void * h, int64_t * out)
 }
 
 // which: 0 sigma0, 1 taumin, 2 tauscale, 3 time_increment, 4 1 once the
-// kernel forms are tuned (else -1), 5 the batch chunk size in cells, 100 + t
-// the form code of cell type t
+// kernel forms are tuned (else -1), 5 the batch chunk size in cells, 6 1 when
+// whole steps replay a captured hipGraph (else 0), 100 + t the form code of
+// cell type t
 double sceuler_get_param(void * h, int which)
 {
     DEuler const & e = *static_cast<DEuler *>(h);
@@ -3091,6 +3247,7 @@ double sceuler_get_param(void * h, int which)
     case 2: return e.tauscale;
     case 4: return e.tuned != 0 ? 1.0 : -1.0;
     case 5: return static_cast<double>(e.chunk_cells);
+    case 6: return static_cast<double>(e.use_graph);
     default:
         if (which >= 100 && which < 116)
         {
@@ -3126,6 +3283,7 @@ int sceuler_set_param(void * h, int which, double v)
                 }
                 e.chunk_cells = static_cast<int64_t>(v);
                 break;
+            case 6: e.use_graph = v != 0.0 ? 1 : 0; break;
             default: scamd::fail(scamd::SC_EVALUE, "EulerCore: read-only or unknown parameter");
             }
         });

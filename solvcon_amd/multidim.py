@@ -330,6 +330,18 @@ class EulerCore(object):
     def batch_chunk(self, cells):
         L.check(L.lib().sceuler_set_param(self._h, 5, float(cells)))
 
+    @property
+    def use_graph(self):
+        """Whether ``march`` and ``march_batch`` replay each whole step
+        (the half-step pair) from a captured hipGraph instead of launching
+        its kernels one by one (default False, or the environment's
+        SC_EULER_GRAPH).  Results do not depend on it."""
+        return L.lib().sceuler_get_param(self._h, 6) != 0.0
+
+    @use_graph.setter
+    def use_graph(self, on):
+        L.check(L.lib().sceuler_set_param(self._h, 6, 1.0 if on else 0.0))
+
     def march_batch(self):
         """March every instance its own step count; returns the device
         time of the march in milliseconds."""

@@ -155,3 +155,26 @@ def test_gpu_batch_guards():
         ec.batch_set_dt([0.1])
     with pytest.raises(ValueError):
         BatchedEuler([d], [st], [0])
+
+
+@pytest.mark.gpu
+def test_gpu_batch_graph_equals_eager():
+    # The batched march replays one captured hipGraph per range of active
+    # ranks (one step each); instances ending at different steps make
+    # several ranges with several replays each.  Every table equals the
+    # batch marched with its kernels launched one by one.
+    print('case: batch graph vs eager', flush=True)
+    from solvcon_amd.batch import (batch_config, config_defs, config_states,
+                                   BatchedEuler)
+    m, ms, rs, steps = batch_config(ninst=6, lo=50, hi=600, smin=3, smax=12)
+    defs = config_defs(m, ms)
+    st = config_states(rs)
+    runs = []
+    for graph in (True, False):
+        b = BatchedEuler(defs, st, steps)
+        b.core.use_graph = graph
+        b.march()
+        runs.append(b)
+    for name in TABLES:
+        assert np.array_equal(getattr(runs[0].core, name).ndarray,
+                              getattr(runs[1].core, name).ndarray), name

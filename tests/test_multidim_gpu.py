@@ -473,3 +473,42 @@ def test_per_type_forms(orc, monkeypatch):
     oe.march(5)
     pair.same_euler(ge, oe)
     assert ge.kernel_forms == want
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager():
+    # Whole steps replayed from a captured hipGraph equal the same steps
+    # launched one by one, across march calls with eager work between them
+    # (downloads, tuning, odd step counts) and after parameter changes that
+    # the captured launches depend on (sigma0, dt).
+    print('case: graph replay vs eager', flush=True)
+    import solvcon_amd as sc
+    g = pair.gpu_mesh(_defs()['mixed3d'])
+
+    def fresh(graph):
+        e = sc.EulerCore(mesh=g, time_increment=2e-3)
+        e.use_graph = graph
+        assert e.use_graph == graph
+        e.init_solution(GAMMA, 1.0, [0.1, 0.0, 0.0], 1.0)
+        st = e.so0n.ndarray
+        st[e.ngstcell:, 0] += np.linspace(0.0, 0.4, e.ncell)
+        del st
+        faces = g.bndfcs.ndarray[:, 0].copy()
+        e.add_nonrefl(faces[0::2])
+        e.add_slipwall(faces[1::2])
+        e.bc_soln()
+        e.bc_dsoln()
+        return e
+    a = fresh(True)
+    b = fresh(False)
+    for k, steps in enumerate((3, 4, 1, 5, 2)):
+        if k == 2:
+            a.sigma0 = b.sigma0 = 2.5
+        if k == 3:
+            a.set_time_increment(1.5e-3)
+            b.set_time_increment(1.5e-3)
+        a.march(steps)
+        b.march(steps)
+        for t in ('so0n', 'so1n', 'so0c', 'so1c', 'so0t', 'cflo', 'cflc'):
+            assert np.array_equal(getattr(a, t).ndarray,
+                                  getattr(b, t).ndarray), (k, t)
