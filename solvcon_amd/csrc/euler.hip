@@ -104,7 +104,7 @@ constexpr int waves_of(int mode)
 // role is compiled in a few variants: a minimum wave count per SIMD (which
 // caps registers) and whether the block's own rows are staged in LDS.  A
 // variant code is waves + 10 * staged.
-constexpr int NVAR = 3;
+constexpr int NVAR = 4;
 
 constexpr int mode_waves(int waves)
 {
@@ -116,9 +116,13 @@ constexpr int role_variant(int nd, int role, int v)
     // 0 marks an unused slot.
     if (nd == 2)
     {
-        return role == 0 ? (v == 0 ? 1 : v == 1 ? 3 : 11) : (v == 0 ? 1 : 0);
+        return role == 0 ? (v == 0 ? 1 : v == 1 ? 3 : v == 2 ? 11 : 0) : (v == 0 ? 1 : 0);
     }
-    return role == 0 ? (v == 0 ? 1 : v == 1 ? 2 : 102) : role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : 12);
+    if (role == 0)
+    {
+        return v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 102 : 112;
+    }
+    return role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 12 : 0);
 }
 
 // Mode bits of a variant code: waves + 10 when LDS-staged + 100 when the
@@ -1856,6 +1860,7 @@ struct DEuler
         case 11: run_group<MODE | variant_mode(11)>(p, gr, cnt); break;
         case 12: run_group<MODE | variant_mode(12)>(p, gr, cnt); break;
         case 102: run_group<MODE | variant_mode(102)>(p, gr, cnt); break;
+        case 112: run_group<MODE | variant_mode(112)>(p, gr, cnt); break;
         default: run_group<MODE>(p, gr, cnt); break;
         }
     }

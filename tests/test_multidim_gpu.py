@@ -225,7 +225,7 @@ def test_march_bitwise(orc, name, sfc):
 # the block's rows are staged in LDS and 't' when the fused kernel reads
 # so0t from memory.
 FORMS_2D = ('f1', 'f3', 'f1s', 's1/1')
-FORMS_3D = ('f1', 'f2', 'f2t', 's1/1', 's1s/1', 's1/2', 's1s/2s')
+FORMS_3D = ('f1', 'f2', 'f2t', 'f2st', 's1/1', 's1s/1', 's1/2', 's1s/2s')
 
 
 @pytest.mark.parametrize('name,form',
