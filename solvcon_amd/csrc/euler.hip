@@ -133,6 +133,15 @@ constexpr int variant_mode(int code)
     return mode_waves(code % 10) | ((code / 10) % 10 != 0 ? M_STAGE : 0) | (code >= 100 ? M_STMEM : 0);
 }
 
+// The batched march runs the fused kernel that recomputes so0t, at three
+// wavefronts per SIMD for triangles, two for 3D cells and one for quads
+// (the fastest such variants of the plain march that do not spill, or that
+// win despite spilling in 3D).
+constexpr int batch_waves(int type)
+{
+    return type == CT_TRI ? M_W3 : type == CT_QUAD ? 0 : M_W2;
+}
+
 constexpr bool variant_built(int nd, int mode)
 {
     bool const soln = (mode & M_SOLN) != 0;
@@ -145,7 +154,7 @@ constexpr bool variant_built(int nd, int mode)
     }
     if ((mode & M_BATCH) != 0)
     {
-        return false;
+        return (mode & vmask) == (nd == 2 ? M_W3 : M_W2);
     }
     for (int v = 0; v < NVAR; ++v)
     {
@@ -1894,12 +1903,12 @@ struct DEuler
             constexpr int MB = M_FUSED | M_BATCH;
             switch (gr.type)
             {
-            case CT_TRI: k_cell<CT_TRI, MB><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_QUAD: k_cell<CT_QUAD, MB><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_TET: k_cell<CT_TET, MB><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_HEX: k_cell<CT_HEX, MB><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_PRISM: k_cell<CT_PRISM, MB><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_PYRAMID: k_cell<CT_PYRAMID, MB><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TRI: k_cell<CT_TRI, MB | batch_waves(CT_TRI)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_QUAD: k_cell<CT_QUAD, MB | batch_waves(CT_QUAD)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TET: k_cell<CT_TET, MB | batch_waves(CT_TET)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_HEX: k_cell<CT_HEX, MB | batch_waves(CT_HEX)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PRISM: k_cell<CT_PRISM, MB | batch_waves(CT_PRISM)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PYRAMID: k_cell<CT_PYRAMID, MB | batch_waves(CT_PYRAMID)><<<nb, BLK, 0, stream>>>(p, g); break;
             default: fail(SC_ERUNTIME, "EulerCore: unsupported cell type in group");
             }
             SC_HIP(hipGetLastError());
