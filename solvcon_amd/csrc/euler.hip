@@ -90,6 +90,7 @@ enum Mode : int
     M_W4 = 256, // four wavefronts per SIMD (at most 128 registers)
     M_WAVES = M_W2 | M_W3 | M_W4,
     M_STAGE = 512, // stage the block's own rows in LDS for neighbour reads
+    M_GDMA = 1024, // prefetch the sub-face geometry into LDS (LDS DMA)
     M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
 };
 
@@ -102,9 +103,11 @@ constexpr int waves_of(int mode)
 
 // Kernel roles: 0 the fused kernel, 1 soln + BC + CFL, 2 dsoln.  Each
 // role is compiled in a few variants: a minimum wave count per SIMD (which
-// caps registers) and whether the block's own rows are staged in LDS.  A
-// variant code is waves + 10 * staged.
-constexpr int NVAR = 4;
+// caps registers), whether the block's own rows are staged in LDS, whether
+// the fused kernel reads so0t from memory and whether the sub-face geometry
+// is prefetched into LDS.  A variant code is waves + 10 * staged + 100 *
+// so0t read + 1000 * geometry prefetch.
+constexpr int NVAR = 5;
 
 constexpr int mode_waves(int waves)
 {
@@ -116,21 +119,23 @@ constexpr int role_variant(int nd, int role, int v)
     // 0 marks an unused slot.
     if (nd == 2)
     {
-        return role == 0 ? (v == 0 ? 1 : v == 1 ? 3 : v == 2 ? 11 : 0) : (v == 0 ? 1 : 0);
+        return role == 0 ? (v == 0 ? 1 : v == 1 ? 3 : v == 2 ? 11 : v == 3 ? 1001 : 1003) : (v == 0 ? 1 : 0);
     }
     if (role == 0)
     {
-        return v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 102 : 112;
+        return v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 102 : v == 3 ? 112 : 0;
     }
     return role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 12 : 0);
 }
 
 // Mode bits of a variant code: waves + 10 when LDS-staged + 100 when the
 // fused kernel reads the neighbours' so0t from memory (calc_solt first)
-// instead of recomputing it.
+// instead of recomputing it + 1000 when the sub-face geometry is prefetched
+// into LDS.
 constexpr int variant_mode(int code)
 {
-    return mode_waves(code % 10) | ((code / 10) % 10 != 0 ? M_STAGE : 0) | (code >= 100 ? M_STMEM : 0);
+    return mode_waves(code % 10) | ((code / 10) % 10 != 0 ? M_STAGE : 0) | ((code / 100) % 10 != 0 ? M_STMEM : 0) |
+           ((code / 1000) % 10 != 0 ? M_GDMA : 0);
 }
 
 // The batched march runs the fused kernel that recomputes so0t, at three
@@ -142,12 +147,30 @@ constexpr int batch_waves(int type)
     return type == CT_TRI ? M_W3 : type == CT_QUAD ? 0 : M_W2;
 }
 
+// LDS bytes per block of the sub-face geometry prefetch (2 S d fields of
+// 64 cells per wave), and whether the blocks that the mode's wave count
+// keeps resident on a CU fit in its 160 KB.
+constexpr int gdma_bytes(int type)
+{
+    return type == CT_TRI ? 4 * 2 * 6 * 2 * 64 * 8 : type == CT_QUAD ? 4 * 2 * 8 * 2 * 64 * 8 : 0;
+}
+
+constexpr bool gdma_fits(int type, int mode)
+{
+    int const w = (mode & M_W4) != 0 ? 4 : (mode & M_W3) != 0 ? 3 : (mode & M_W2) != 0 ? 2 : 1;
+    return gdma_bytes(type) > 0 && w * gdma_bytes(type) <= 160 * 1024;
+}
+
 constexpr bool variant_built(int nd, int mode)
 {
     bool const soln = (mode & M_SOLN) != 0;
     bool const dsoln = (mode & M_DSOLN) != 0;
     int const role = soln && dsoln ? 0 : soln ? 1 : 2;
-    int const vmask = M_WAVES | M_STAGE | (role == 0 ? M_STMEM : 0);
+    if (role != 0 && (mode & M_GDMA) != 0)
+    {
+        return false;
+    }
+    int const vmask = M_WAVES | M_STAGE | (role == 0 ? M_STMEM | M_GDMA : 0);
     if ((mode & vmask) == 0)
     {
         return true;
@@ -258,6 +281,36 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     constexpr int O_CN = O_GM + (STG_GM ? BLK : 0);
     constexpr int LW = STAGE ? O_CN + BLK * ND : 1;
     __shared__ double lds[LW];
+    // Sub-face geometry prefetch: each wave copies the sub-face centroids
+    // and normals of its 64 cells into its own LDS region with LDS-DMA
+    // loads of 16 B per lane (lanes 0-31 one field, lanes 32-63 the next,
+    // two cells each), so the geometry arrives in one batch while the
+    // neighbour rows are gathered.  Field G0 + j of lane l's cell is then
+    // gw[j * 64 + l].  The host launches it only when the field stride is
+    // even and the table 16-byte aligned, so every load is aligned; a lane
+    // reads at most one cell past the group's count, which is still inside
+    // the next field.
+    constexpr bool GDMA = (MODE & M_GDMA) != 0 && (MODE & M_SOLN) != 0;
+    constexpr int G0 = LY::FC;
+    constexpr int GN = 2 * TR::S * ND;
+    constexpr int GW = GDMA ? GN * 64 : 1;
+    __shared__ double gsh[GDMA ? (BLK / 64) * GW : 1];
+    int const lane = threadIdx.x & 63;
+    double * const gw = gsh + (GDMA ? (threadIdx.x >> 6) * GW : 0);
+    if constexpr (GDMA)
+    {
+        int64_t const c = i0 + (threadIdx.x & ~63u) + 2 * (lane & 31);
+        if (c < g.cnt)
+        {
+#pragma unroll
+            for (int q = 0; q < GN / 2; ++q)
+            {
+                int64_t const fld = G0 + 2 * q + (lane >> 5);
+                __builtin_amdgcn_global_load_lds((void const __attribute__((address_space(1))) *)(g.geo + fld * g.n + c),
+                                                 (void __attribute__((address_space(3))) *)(gw + q * 128), 16, 0, 0);
+            }
+        }
+    }
     int64_t const rbase = p.G + g.off + i0;
     int const nrow = static_cast<int>(g.cnt - i0 < BLK ? g.cnt - i0 : BLK);
     if constexpr (STAGE)
@@ -297,6 +350,18 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     int64_t const n = g.n;
     int64_t const row = p.G + g.off + i;
     double const * geo = g.geo + i;
+    // One sub-face geometry field of this cell, from the prefetch or HBM.
+    auto gsub = [&](int fld) -> double
+    {
+        if constexpr (GDMA)
+        {
+            return gw[(fld - G0) * 64 + lane];
+        }
+        else
+        {
+            return geo[fld * n];
+        }
+    };
     double hdt = p.hdt;
     double qdt = p.qdt;
     if constexpr ((MODE & M_BATCH) != 0)
@@ -338,6 +403,12 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 
     if constexpr ((MODE & M_SOLN) != 0)
     {
+        if constexpr (GDMA)
+        {
+            // The compiler does not track LDS-DMA loads: wait for them here,
+            // and let no LDS read move above the wait.
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
         // calc_soln_impl (euler_march.cpp:219-302).
         double acc[NEQ];
 #pragma unroll
@@ -407,7 +478,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
                     {
-                        usfc[s][k] += (geo[(LY::FC + sf * ND + d) * n] - jce[f][d]) * j1[k][d];
+                        usfc[s][k] += (gsub(LY::FC + sf * ND + d) - jce[f][d]) * j1[k][d];
                     }
                 }
             }
@@ -430,7 +501,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                         {
                             dfcn += jr[m][d2] * usfc[s][m];
                         }
-                        dot += dfcn * geo[(LY::FN + sf * ND + d2) * n];
+                        dot += dfcn * gsub(LY::FN + sf * ND + d2);
                     }
                     acc[k] -= hdt * dot;
                 }
@@ -1822,7 +1893,7 @@ struct DEuler
     template <int T, int MODE>
     void launch_cell(KP const & p, GV const & g, unsigned nb)
     {
-        if constexpr (variant_built(Traits<T>::ND, MODE))
+        if constexpr (variant_built(Traits<T>::ND, MODE) && ((MODE & M_GDMA) == 0 || gdma_fits(T, MODE)))
         {
             k_cell<T, MODE><<<nb, BLK, 0, stream>>>(p, g);
         }
@@ -1857,11 +1928,25 @@ struct DEuler
         SC_HIP(hipGetLastError());
     }
 
+    // The geometry prefetch issues 16-byte loads of two cells each: it needs
+    // an even field stride, a 16-byte aligned table and room in LDS for the
+    // blocks the variant keeps resident.
+    static bool has_gdma(int code) { return (code / 1000) % 10 != 0; }
+    static bool gdma_ok(Group const & gr, int code)
+    {
+        return gr.n % 2 == 0 && reinterpret_cast<uintptr_t>(gr.geo.ptr) % 16 == 0 &&
+               gdma_fits(gr.type, variant_mode(code));
+    }
+
     // run_group over a whole group with the variant chosen at run time.
     template <int MODE>
     void run_group_v(KP const & p, Group const & gr, int code, bool warm = false)
     {
         int64_t const cnt = warm ? -1 : gr.n;
+        if (has_gdma(code) && !gdma_ok(gr, code))
+        {
+            code -= 1000; // the same variant without the geometry prefetch
+        }
         switch (code)
         {
         case 2: run_group<MODE | variant_mode(2)>(p, gr, cnt); break;
@@ -1870,6 +1955,8 @@ struct DEuler
         case 12: run_group<MODE | variant_mode(12)>(p, gr, cnt); break;
         case 102: run_group<MODE | variant_mode(102)>(p, gr, cnt); break;
         case 112: run_group<MODE | variant_mode(112)>(p, gr, cnt); break;
+        case 1001: run_group<MODE | variant_mode(1001)>(p, gr, cnt); break;
+        case 1003: run_group<MODE | variant_mode(1003)>(p, gr, cnt); break;
         default: run_group<MODE>(p, gr, cnt); break;
         }
     }
@@ -2106,7 +2193,8 @@ struct DEuler
     }
 
     // Whether a form reads so0t from memory, so calc_solt must run first.
-    static bool needs_solt(Form const & f) { return f.split != 0 || f.vf >= 100; }
+    static bool reads_solt(int code) { return (code / 100) % 10 != 0; }
+    static bool needs_solt(Form const & f) { return f.split != 0 || reads_solt(f.vf); }
 
     void run_role(KP const & p, Group const & gr, int role, int code, bool warm = false)
     {
@@ -2162,7 +2250,7 @@ struct DEuler
                 for (int v = 0; v < NVAR; ++v)
                 {
                     int const c = role_variant(ndim, role, v);
-                    if (c != 0)
+                    if (c != 0 && (!has_gdma(c) || gdma_ok(groups[gi], c)))
                     {
                         run_role(p, groups[gi], role, c, true);
                     }
@@ -2181,7 +2269,7 @@ struct DEuler
                 for (int v = 0; v < NVAR; ++v)
                 {
                     int const c = role_variant(ndim, role, v);
-                    if (c == 0)
+                    if (c == 0 || (has_gdma(c) && !gdma_ok(groups[gi], c)))
                     {
                         continue;
                     }
@@ -2225,8 +2313,8 @@ struct DEuler
                         int const c = role_variant(ndim, role, v);
                         if (t >= 0.0f)
                         {
-                            std::fprintf(stderr, "  w%d%s%s %.4f ms", c % 10, (c / 10) % 10 != 0 ? "+lds" : "",
-                                         c >= 100 ? "+so0t" : "", t);
+                            std::fprintf(stderr, "  w%d%s%s%s %.4f ms", c % 10, (c / 10) % 10 != 0 ? "+lds" : "",
+                                         (c / 100) % 10 != 0 ? "+so0t" : "", (c / 1000) % 10 != 0 ? "+gdma" : "", t);
                         }
                     }
                     std::fprintf(stderr, "\n");
@@ -2255,11 +2343,11 @@ struct DEuler
                     }
                     if (role == 0)
                     {
-                        double & b = c >= 100 ? tft[gi] : tf[gi];
+                        double & b = reads_solt(c) ? tft[gi] : tf[gi];
                         if (b < 0.0 || t < b)
                         {
                             b = t;
-                            (c >= 100 ? fastt[gi] : fast[gi]).vf = c;
+                            (reads_solt(c) ? fastt[gi] : fast[gi]).vf = c;
                         }
                     }
                     else if (best[role] < 0.0 || t < best[role])
@@ -2315,7 +2403,11 @@ struct DEuler
             if (groups[gi].type == type)
             {
                 Form const & f = gform[gi];
-                return f.split != 0 ? 10000 + 100 * f.v1 + f.v2 : f.vf;
+                if (f.split != 0)
+                {
+                    return 10000 + 100 * f.v1 + f.v2;
+                }
+                return has_gdma(f.vf) && !gdma_ok(groups[gi], f.vf) ? f.vf - 1000 : f.vf;
             }
         }
         return 0;
@@ -2596,6 +2688,11 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
             if (k < t.size() && t[k] == 't')
             {
                 code += 100;
+                ++k;
+            }
+            if (k < t.size() && t[k] == 'g')
+            {
+                code += 1000;
                 ++k;
             }
             return k == t.size();

@@ -183,14 +183,16 @@ class EulerCore(object):
         fused kernel in variant V, or 'sA/B', the split pair (soln + BC +
         CFL, then dsoln) in variants A and B.  A variant is the minimum
         wavefronts per SIMD it is compiled for, followed by 's' when the
-        block's own rows are staged in LDS and 't' when the fused kernel
-        reads the neighbours' so0t from memory."""
+        block's own rows are staged in LDS, 't' when the fused kernel
+        reads the neighbours' so0t from memory and 'g' when it prefetches
+        the sub-face geometry into LDS."""
         if L.lib().sceuler_get_param(self._h, 4) < 0:
             return None
 
         def var(c):
-            return '%d%s%s' % (c % 10, 's' if (c // 10) % 10 else '',
-                               't' if c >= 100 else '')
+            return '%d%s%s%s' % (c % 10, 's' if (c // 10) % 10 else '',
+                                 't' if (c // 100) % 10 else '',
+                                 'g' if (c // 1000) % 10 else '')
         types = np.unique(self.mesh.cltpn.ndarray[self.mesh.ngstcell:])
         out = {}
         for t in types:

@@ -21,6 +21,7 @@ def _defs():
     from solvcon_amd.gen import tri_mesh_def, mixed_mesh_def
     return {
         'tri16': tri_mesh_def(16, seed=2),
+        'tri15': tri_mesh_def(15, seed=5),
         'channel': MD.quad_channel(6, 3, 4.0, 2.0),
         'hexblk': MD.hex_block(4, 3, 3, jitter=0.2, seed=5),
         'mixed3d': mixed_mesh_def(6, seed=3),
@@ -222,14 +223,16 @@ def test_march_bitwise(orc, name, sfc):
 
 # Every compiled kernel variant: fV the fused kernel in variant V, sA/B the
 # split pair; a variant is the minimum wavefronts per SIMD, with 's' when
-# the block's rows are staged in LDS and 't' when the fused kernel reads
-# so0t from memory.
-FORMS_2D = ('f1', 'f3', 'f1s', 's1/1')
+# the block's rows are staged in LDS, 't' when the fused kernel reads so0t
+# from memory and 'g' when it prefetches the sub-face geometry into LDS
+# (tri15: 450 cells, so the last wave is partial).
+FORMS_2D = ('f1', 'f3', 'f1s', 'f1g', 'f3g', 's1/1')
 FORMS_3D = ('f1', 'f2', 'f2t', 'f2st', 's1/1', 's1s/1', 's1/2', 's1s/2s')
 
 
 @pytest.mark.parametrize('name,form',
                          [(n, f) for n in ('tri16', 'channel') for f in FORMS_2D]
+                         + [('tri15', f) for f in ('f1g', 'f3g')]
                          + [('mixed3d', f) for f in FORMS_3D])
 def test_march_every_kernel_form(orc, monkeypatch, name, form):
     # The autotuner may pick any variant per cell type; each fixed variant
@@ -249,7 +252,13 @@ def test_march_every_kernel_form(orc, monkeypatch, name, form):
     ge.march(7)
     oe.march(7)
     pair.same_euler(ge, oe)
-    assert set(ge.kernel_forms.values()) == {form}
+    got = set(ge.kernel_forms.values())
+    if form.endswith('g') and name == 'channel':
+        # The quads' prefetch needs 64 KB of LDS per block, too much for
+        # three resident blocks; such a group runs the variant without it.
+        assert got <= {form, form[:-1]}
+    else:
+        assert got == {form}
 
 
 def test_autotuner_reports_forms(orc, monkeypatch):
