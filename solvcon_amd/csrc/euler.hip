@@ -177,7 +177,7 @@ constexpr bool variant_built(int nd, int mode)
     }
     if ((mode & M_BATCH) != 0)
     {
-        return (mode & vmask) == (nd == 2 ? M_W3 : M_W2);
+        return (mode & (vmask & ~M_GDMA)) == (nd == 2 ? M_W3 : M_W2) && (nd == 2 || (mode & M_GDMA) == 0);
     }
     for (int v = 0; v < NVAR; ++v)
     {
@@ -1988,9 +1988,22 @@ struct DEuler
             GV const g{gr.off + lo, gr.n, gr.nbr.ptr + lo, gr.geo.ptr + lo, cnt};
             unsigned const nb = grid_for(cnt, BLK);
             constexpr int MB = M_FUSED | M_BATCH;
+            constexpr int MT = MB | batch_waves(CT_TRI);
+            // Triangles prefetch their sub-face geometry when the range
+            // starts at an even cell (always, unless rank chunks split it).
+            bool const tri_gdma = lo % 2 == 0 && gdma_ok(gr, 1003);
             switch (gr.type)
             {
-            case CT_TRI: k_cell<CT_TRI, MB | batch_waves(CT_TRI)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TRI:
+                if (tri_gdma)
+                {
+                    k_cell<CT_TRI, MT | M_GDMA><<<nb, BLK, 0, stream>>>(p, g);
+                }
+                else
+                {
+                    k_cell<CT_TRI, MT><<<nb, BLK, 0, stream>>>(p, g);
+                }
+                break;
             case CT_QUAD: k_cell<CT_QUAD, MB | batch_waves(CT_QUAD)><<<nb, BLK, 0, stream>>>(p, g); break;
             case CT_TET: k_cell<CT_TET, MB | batch_waves(CT_TET)><<<nb, BLK, 0, stream>>>(p, g); break;
             case CT_HEX: k_cell<CT_HEX, MB | batch_waves(CT_HEX)><<<nb, BLK, 0, stream>>>(p, g); break;
