@@ -123,7 +123,7 @@ constexpr int role_variant(int nd, int role, int v)
     }
     if (role == 0)
     {
-        return v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 102 : v == 3 ? 112 : 0;
+        return v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 102 : v == 3 ? 112 : 1102;
     }
     return role == 1 ? (v == 0 ? 1 : v == 1 ? 11 : 0) : (v == 0 ? 1 : v == 1 ? 2 : v == 2 ? 12 : 0);
 }
@@ -147,12 +147,19 @@ constexpr int batch_waves(int type)
     return type == CT_TRI ? M_W3 : type == CT_QUAD ? 0 : M_W2;
 }
 
-// LDS bytes per block of the sub-face geometry prefetch (2 S d fields of
-// 64 cells per wave), and whether the blocks that the mode's wave count
-// keeps resident on a CU fit in its 160 KB.
+// Fields of the sub-face geometry prefetch: the sub-face centroids and
+// normals in 2D (2 S d fields), the centroids alone in 3D (S d).  LDS bytes
+// per block (64 cells per wave, four waves), and whether the blocks that the
+// mode's wave count keeps resident on a CU fit in its 160 KB.
+constexpr int gdma_fields(int type)
+{
+    return type == CT_TRI ? 2 * 6 * 2 : type == CT_QUAD ? 2 * 8 * 2 : type == CT_TET ? 12 * 3 : type == CT_HEX ? 24 * 3
+           : type == CT_PRISM ? 18 * 3 : type == CT_PYRAMID ? 16 * 3 : 0;
+}
+
 constexpr int gdma_bytes(int type)
 {
-    return type == CT_TRI ? 4 * 2 * 6 * 2 * 64 * 8 : type == CT_QUAD ? 4 * 2 * 8 * 2 * 64 * 8 : 0;
+    return 4 * gdma_fields(type) * 64 * 8;
 }
 
 constexpr bool gdma_fits(int type, int mode)
@@ -282,9 +289,9 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     constexpr int LW = STAGE ? O_CN + BLK * ND : 1;
     __shared__ double lds[LW];
     // Sub-face geometry prefetch: each wave copies the sub-face centroids
-    // and normals of its 64 cells into its own LDS region with LDS-DMA
-    // loads of 16 B per lane (lanes 0-31 one field, lanes 32-63 the next,
-    // two cells each), so the geometry arrives in one batch while the
+    // (and in 2D the normals) of its 64 cells into its own LDS region with
+    // LDS-DMA loads of 16 B per lane (lanes 0-31 one field, lanes 32-63 the
+    // next, two cells each), so the geometry arrives in one batch while the
     // neighbour rows are gathered.  Field G0 + j of lane l's cell is then
     // gw[j * 64 + l].  The host launches it only when the field stride is
     // even and the table 16-byte aligned, so every load is aligned; a lane
@@ -292,7 +299,8 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     // the next field.
     constexpr bool GDMA = (MODE & M_GDMA) != 0 && (MODE & M_SOLN) != 0;
     constexpr int G0 = LY::FC;
-    constexpr int GN = 2 * TR::S * ND;
+    constexpr int GN = gdma_fields(T);
+    static_assert(!GDMA || (GN == (ND == 2 ? 2 : 1) * TR::S * ND && GN % 2 == 0), "prefetched fields");
     constexpr int GW = GDMA ? GN * 64 : 1;
     __shared__ double gsh[GDMA ? (BLK / 64) * GW : 1];
     int const lane = threadIdx.x & 63;
@@ -355,12 +363,12 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     {
         if constexpr (GDMA)
         {
-            return gw[(fld - G0) * 64 + lane];
+            if (fld < G0 + GN)
+            {
+                return gw[(fld - G0) * 64 + lane];
+            }
         }
-        else
-        {
-            return geo[fld * n];
-        }
+        return geo[fld * n];
     };
     double hdt = p.hdt;
     double qdt = p.qdt;
@@ -1957,6 +1965,7 @@ struct DEuler
         case 112: run_group<MODE | variant_mode(112)>(p, gr, cnt); break;
         case 1001: run_group<MODE | variant_mode(1001)>(p, gr, cnt); break;
         case 1003: run_group<MODE | variant_mode(1003)>(p, gr, cnt); break;
+        case 1102: run_group<MODE | variant_mode(1102)>(p, gr, cnt); break;
         default: run_group<MODE>(p, gr, cnt); break;
         }
     }
