@@ -7,19 +7,23 @@
  * Every kernel evaluates the reference arithmetic in the reference order
  * (built with -ffp-contract=off), so results match the CPU march bit for bit.
  *
- * Two march paths:
- *  - n <= LDS_MAX_N: one persistent workgroup keeps so0/so1 in LDS and runs
- *    every half step with workgroup barriers in between (the Sod-tube size,
- *    2001 points, is launch-latency bound, so one launch per march call).
- *  - larger n: one fused kernel per half step (so0, cfl and so1 together;
- *    each thread owns one new solution point) plus a one-thread boundary
- *    kernel between the two half steps.
+ * March paths:
+ *  - by default the tiled march (k_march_tiled): segments of the tube with
+ *    halos in LDS, up to 64 steps per launch and no synchronisation between
+ *    blocks, so a short tube spreads over many CUs instead of one;
+ *  - with SC_1D_TILED=0, for n <= LDS_MAX_N one persistent workgroup keeps
+ *    so0/so1 in LDS and runs every half step with workgroup barriers in
+ *    between, and for larger n one fused kernel runs per half step (so0,
+ *    cfl and so1 together; each thread owns one new solution point) plus a
+ *    one-thread boundary kernel between the two half steps.
  */
 
 #include "common.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <memory>
+#include <utility>
 
 namespace scamd
 {
@@ -361,6 +365,107 @@ __global__ void __launch_bounds__(PBLOCK) k_march_lds(int64_t n, int64_t steps, 
     }
 }
 
+// Temporal blocking of the march (march_alpha, Euler1DCore.hpp:339-349):
+// block b owns rows [b seg, (b + 1) seg) of [0, n), copies the rows within
+// `halo` of them from (in0, in1) into LDS and runs `steps` whole steps there
+// without synchronising with other blocks, then writes its own rows to
+// (out0, out1).  A row is updated only when both its neighbours lie in the
+// block's window; the stale values this leaves next to an inner window edge
+// spread by one row per half step, so with halo >= 2 steps + 2 the owned
+// rows are exact.  The boundary rows 1 and n - 2 are copied after the first
+// half of every step by the block whose window holds them.  With `last`,
+// the CFL of the last step is written for the owned rows.  Input and output
+// are different buffers: other blocks read their halos from the input.
+constexpr int TBLOCK = 512;
+constexpr int TROWS = 1024; // most rows of one window (48 KB of LDS)
+
+template <int ALPHA>
+__global__ void __launch_bounds__(TBLOCK) k_march_tiled(int64_t n, int64_t seg, int64_t halo, int64_t steps, int last,
+                                                        double hdt, double qdt, double const * coord,
+                                                        double const * in0, double const * in1, double * out0,
+                                                        double * out1, double const * gamma, double * cfl)
+{
+    __shared__ double s0[TROWS * 3];
+    __shared__ double s1[TROWS * 3];
+    int64_t const own0 = static_cast<int64_t>(blockIdx.x) * seg;
+    int64_t const own1 = own0 + seg < n ? own0 + seg : n;
+    int64_t const lo = own0 - halo > 0 ? own0 - halo : 0;
+    int64_t const hi = own1 + halo < n ? own1 + halo : n;
+    int const tid = threadIdx.x;
+    for (int64_t q = tid; q < (hi - lo) * 3; q += blockDim.x)
+    {
+        s0[q] = in0[lo * 3 + q];
+        s1[q] = in1[lo * 3 + q];
+    }
+    __syncthreads();
+    for (int64_t it = 0; it < steps; ++it)
+    {
+        for (int half = 0; half < 2; ++half)
+        {
+            Geom1D const p = plane(n, half == 1);
+            // Left elements ic of this plane whose rows ic .. ic + 2 lie in
+            // the window.
+            int64_t k0 = lo > p.start ? (lo - p.start + 1) / 2 : 0;
+            int64_t k1 = hi - 3 >= p.start ? (hi - 3 - p.start) / 2 + 1 : 0;
+            k1 = k1 < p.nout ? k1 : p.nout;
+            for (int64_t k = k0 + tid; k < k1; k += blockDim.x)
+            {
+                int64_t const ic = p.start + 2 * k;
+                int64_t const l = ic - lo;
+                Elem xn;
+                Elem xp;
+                double const gn = gamma[ic];
+                {
+                    double u[3] = {s0[l * 3], s0[l * 3 + 1], s0[l * 3 + 2]};
+                    double ux[3] = {s1[l * 3], s1[l * 3 + 1], s1[l * 3 + 2]};
+                    derive(xn, gn, hdt, coord[ic], coord[ic - 1], coord[ic + 1], u, ux);
+                }
+                {
+                    int64_t const m = l + 2;
+                    double u[3] = {s0[m * 3], s0[m * 3 + 1], s0[m * 3 + 2]};
+                    double ux[3] = {s1[m * 3], s1[m * 3 + 1], s1[m * 3 + 2]};
+                    derive(xp, gamma[ic + 2], hdt, coord[ic + 2], coord[ic + 1], coord[ic + 3], u, ux);
+                }
+                double n0[3];
+                new_so0(xn, xp, hdt, qdt, coord[ic + 2] - coord[ic], n0);
+                if (last != 0 && it + 1 == steps && ic >= own0 && ic < own1)
+                {
+                    cfl[ic] = cfl_at(gn, hdt, xn.u[0], xn.u[1], xn.u[2], xn.xneg, xn.x, xn.xpos);
+                }
+                double n1[3];
+                new_so1<ALPHA>(xn, xp, n0, n1);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    s0[(l + 1) * 3 + i] = n0[i];
+                    s1[(l + 1) * 3 + i] = n1[i];
+                }
+            }
+            __syncthreads();
+            if (half == 0)
+            {
+                if (tid < 3 && lo <= 1 && hi > 2)
+                {
+                    s0[(1 - lo) * 3 + tid] = s0[(2 - lo) * 3 + tid];
+                    s1[(1 - lo) * 3 + tid] = s1[(2 - lo) * 3 + tid];
+                }
+                else if (tid >= 64 && tid < 67 && lo <= n - 3 && hi > n - 2)
+                {
+                    int const i = tid - 64;
+                    s0[(n - 2 - lo) * 3 + i] = s0[(n - 3 - lo) * 3 + i];
+                    s1[(n - 2 - lo) * 3 + i] = s1[(n - 3 - lo) * 3 + i];
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int64_t q = tid; q < (own1 - own0) * 3; q += blockDim.x)
+    {
+        out0[own0 * 3 + q] = s0[(own0 - lo) * 3 + q];
+        out1[own0 * 3 + q] = s1[(own0 - lo) * 3 + q];
+    }
+}
+
 // Derived fields (Euler1DCore.hpp:106-168, Euler1DCore.cpp:37-68,174-205).
 __global__ void k_derived(int which, int64_t n, double const * so0, double const * gamma, double * out)
 {
@@ -406,11 +511,16 @@ struct Core1D
     StreamHolder sh; // declared before the buffers: freed last
     hipStream_t stream = sh.s;
     DevBuf<double> coord, cfl, so0, so1, gamma;
+    DevBuf<double> tso0, tso1; // the other buffers of the tiled march
 
     Core1D(int64_t ncoord, double time_increment)
         : n(ncoord)
         , dt(time_increment)
     {
+        if (char const * env = std::getenv("SC_1D_TILED"))
+        {
+            tiled = std::atoi(env) != 0 ? 1 : 0;
+        }
         AllocScope const scope(stream);
         coord.alloc(n);
         cfl.alloc(n);
@@ -515,10 +625,63 @@ struct Core1D
         SC_HIP(hipGetLastError());
     }
 
+    // The tiled march: segments of 64-512 rows (about n / 128), up to 64
+    // steps per launch with a halo of 2 steps + 2 rows, the state moving
+    // between (so0, so1) and (tso0, tso1) from launch to launch.
+    void march_tiled(int alpha, int64_t steps)
+    {
+        int64_t seg = (n + 127) / 128;
+        seg = seg < 64 ? 64 : seg > 512 ? 512 : seg;
+        int64_t const kmax = (TROWS - seg - 4) / 4; // seg + 2 (2 k + 2) <= TROWS
+        int64_t const kstep = kmax < 64 ? kmax : 64;
+        unsigned const nb = static_cast<unsigned>((n + seg - 1) / seg);
+        if (tso0.ptr == nullptr)
+        {
+            AllocScope const scope(stream);
+            tso0.alloc(n * 3);
+            tso1.alloc(n * 3);
+        }
+        double * a0 = so0.ptr;
+        double * a1 = so1.ptr;
+        double * b0 = tso0.ptr;
+        double * b1 = tso1.ptr;
+        for (int64_t done = 0; done < steps;)
+        {
+            int64_t const k = steps - done < kstep ? steps - done : kstep;
+            int const last = done + k == steps ? 1 : 0;
+            if (alpha == 1)
+            {
+                k_march_tiled<1><<<nb, TBLOCK, 0, stream>>>(n, seg, 2 * k + 2, k, last, hdt(), qdt(), coord.ptr, a0,
+                                                            a1, b0, b1, gamma.ptr, cfl.ptr);
+            }
+            else
+            {
+                k_march_tiled<2><<<nb, TBLOCK, 0, stream>>>(n, seg, 2 * k + 2, k, last, hdt(), qdt(), coord.ptr, a0,
+                                                            a1, b0, b1, gamma.ptr, cfl.ptr);
+            }
+            SC_HIP(hipGetLastError());
+            std::swap(a0, b0);
+            std::swap(a1, b1);
+            done += k;
+        }
+        if (a0 != so0.ptr)
+        {
+            SC_HIP(hipMemcpyAsync(so0.ptr, a0, so0.bytes(), hipMemcpyDeviceToDevice, stream));
+            SC_HIP(hipMemcpyAsync(so1.ptr, a1, so1.bytes(), hipMemcpyDeviceToDevice, stream));
+        }
+    }
+
+    int tiled = 1; // march with march_tiled (else the single-workgroup or per-half-step kernels)
+
     void march(int alpha, int64_t steps)
     {
         if (steps <= 0)
         {
+            return;
+        }
+        if (tiled != 0)
+        {
+            march_tiled(alpha, steps);
             return;
         }
         if (n <= LDS_MAX_N)

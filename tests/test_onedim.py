@@ -176,6 +176,30 @@ def test_gpu_large_bitwise(orc, ncoord):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('ncoord,steps,alpha', [
+    (5, 3, 2), (7, 65, 2), (129, 64, 1), (2001, 1, 2), (2001, 63, 1),
+    (2001, 130, 2), (9001, 65, 2), (9001, 129, 1)])
+@pytest.mark.parametrize('tiled', [1, 0])
+def test_gpu_tiled_bitwise(orc, monkeypatch, ncoord, steps, alpha, tiled):
+    # The tiled march (segments with halos, up to 64 steps per launch) and
+    # the single-workgroup / per-half-step kernels both equal the oracle:
+    # one and several launches, odd and even launch counts, the window
+    # holding the whole tube or a segment of it.
+    print('case: 1D tiled', tiled, ncoord, 'points', steps, 'steps, alpha',
+          alpha, flush=True)
+    monkeypatch.setenv('SC_1D_TILED', str(tiled))
+    dt = 0.4 / 600 * 2000 / (ncoord - 1)
+    st = _gpu_sod(ncoord, 0.4 / dt)
+    o = orc.sod_setup(ncoord=ncoord, time_increment=st.svr.time_increment)
+    getattr(st.svr, 'march_alpha%d' % alpha)(steps)
+    o.march(alpha, steps)
+    assert np.isfinite(o.so0).all()
+    assert (st.svr.so0 == o.so0).all()
+    assert (st.svr.so1 == o.so1).all()
+    assert (st.svr.cfl == o.cfl).all()
+
+
+@pytest.mark.gpu
 def test_gpu_fine_grained_api(orc):
     print('case: fine-grained 1D API', flush=True)
     from solvcon_amd.onedim import Euler1DSolver
