@@ -82,12 +82,17 @@ def _check_batch(defs, states, steps, chunk=0):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('fans', [0, 1, 2])
 @pytest.mark.parametrize('chunk', [0, 150])
-def test_gpu_batch_equals_alone_2d(chunk):
+def test_gpu_batch_equals_alone_2d(chunk, fans):
     # chunk 150: rank chunks of about 150 cells, marched one after another.
-    print('case: batch of 6 2D instances, chunk', chunk, flush=True)
+    print('case: batch of 2D instances, chunk', chunk, 'fans', fans,
+          flush=True)
     from solvcon_amd.gen import riemann_box, quadrant_state
     from solvcon_amd.batch import build_mesh
+    # 3-triangle fans: with one the triangle count is odd and no range runs
+    # the geometry prefetch; with two it is even, and with chunks some rank
+    # ranges start at an odd cell and run without it.
     defs = _defs2d()
     states = []
     for k, d in enumerate(defs):
@@ -95,7 +100,10 @@ def test_gpu_batch_equals_alone_2d(chunk):
         cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
         rho, p = riemann_box(30 + k)
         states.append(quadrant_state(cen / cen.max(0), rho, p))
-    _check_batch(defs, states, [4, 7, 2, 7, 5, 1], chunk)
+    for k in range(fans):
+        defs.append(MD.TRIANGLES)
+        states.append(np.tile([1.0, 0.1 * (k + 1), 0.0, 2.5], (3, 1)))
+    _check_batch(defs, states, [4, 7, 2, 7, 5, 1, 6, 3][:len(defs)], chunk)
 
 
 @pytest.mark.gpu
