@@ -191,23 +191,23 @@ def run_mesh(args, which):
         (time.perf_counter() - t0, ec.ncell, dt))
     ec.march(args.warmup)
     _lib.check(_lib.lib().sc_device_sync())
+    # The timed region: wall clock around the march, and HIP events on the
+    # solver's stream around every substep of the same march for the
+    # kernel time per substep.
     t0 = time.perf_counter()
-    ec.march(args.steps)
+    sub_ms, tot_ms = ec.march_timed(args.steps)
     _lib.check(_lib.lib().sc_device_sync())
     el = time.perf_counter() - t0
     ncell = ec.ncell
     value = ncell * args.steps / el
     log('bench: %d steps in %.3f s, %.4g cell-updates/s' %
         (args.steps, el, value))
-    # Kernel time per substep from HIP events on the solver's stream.
-    nprof = max(10, min(args.steps, 100))
-    sub_ms, tot_ms = ec.march_timed(nprof)
     mix = group_mix(ec)
     b_sub = sum(b_substep(t) * n for t, n in mix.items())  # bytes/substep
     achieved = b_sub / (sub_ms * 1e-3) / 1e9
-    log('bench: fused substep kernels %.4f ms avg (%d substeps), %.1f GB/s '
-        'algorithmic; whole march by events %.4f ms per step'
-        % (sub_ms, 2 * nprof, achieved, tot_ms / nprof))
+    log('bench: substep kernels %.4f ms avg over the %d substeps of the '
+        'timed region, %.1f GB/s algorithmic; whole march by events %.4f ms '
+        'per step' % (sub_ms, 2 * args.steps, achieved, tot_ms / args.steps))
     copy_gbs = None
     if args.copy_probe > 0:
         import ctypes
