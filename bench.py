@@ -515,7 +515,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; --gpus %d is not '
-              'supported (every GPU box here has one GPU)' % args.gpus,
+              'supported (no multi-GPU path is built: the sharded design is '
+              'described, not implemented)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.form:
