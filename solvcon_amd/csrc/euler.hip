@@ -2259,11 +2259,6 @@ struct DEuler
         KP const p = params();
         size_t const ng = groups.size();
         size_t const nslot = ng * 3 * NVAR;
-        std::vector<hipEvent_t> ev(2 + 2 * nslot, nullptr);
-        for (auto & x : ev)
-        {
-            SC_HIP(hipEventCreate(&x));
-        }
         // Load every variant's code first (one empty block each).
         calc_solt();
         for (size_t gi = 0; gi < ng; ++gi)
@@ -2285,21 +2280,25 @@ struct DEuler
         // variant, a kernel meets that variant's cache contents, and
         // variants a few percent apart would swap places from run to run.
         constexpr int TUNE_RUNS = 4;
-        float solt = -1.0f;
-        std::vector<float> tm(nslot, -1.0f);
-        auto keep = [](float & best, float t) { best = best < 0.0f ? t : std::min(best, t); };
+        // Event pairs by (slot, run): slot 0 times calc_solt, slot 1 + sl
+        // the variant of timing slot sl.  Everything is enqueued first and
+        // the stream synchronised once.
+        std::vector<hipEvent_t> ev(2 * TUNE_RUNS * (1 + nslot), nullptr);
+        auto evp = [&](size_t slot, int r, int b) -> hipEvent_t & { return ev[(slot * TUNE_RUNS + r) * 2 + b]; };
+        auto rec = [&](size_t slot, int r, int b)
+        {
+            hipEvent_t & e = evp(slot, r, b);
+            if (e == nullptr)
+            {
+                SC_HIP(hipEventCreate(&e));
+            }
+            SC_HIP(hipEventRecord(e, stream));
+        };
         for (int r = 0; r < TUNE_RUNS; ++r)
         {
-            SC_HIP(hipEventRecord(ev[0], stream));
+            rec(0, r, 0);
             calc_solt();
-            SC_HIP(hipEventRecord(ev[1], stream));
-            SC_HIP(hipEventSynchronize(ev[1]));
-            float t = 0.0f;
-            SC_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
-            if (r > 0)
-            {
-                keep(solt, t);
-            }
+            rec(0, r, 1);
         }
         for (size_t gi = 0; gi < ng; ++gi)
         {
@@ -2315,23 +2314,40 @@ struct DEuler
                     size_t const sl = (gi * 3 + role) * NVAR + v;
                     for (int r = 0; r < TUNE_RUNS; ++r)
                     {
-                        SC_HIP(hipEventRecord(ev[2 + 2 * sl], stream));
+                        rec(1 + sl, r, 0);
                         run_role(p, groups[gi], role, c);
-                        SC_HIP(hipEventRecord(ev[3 + 2 * sl], stream));
-                        SC_HIP(hipEventSynchronize(ev[3 + 2 * sl]));
-                        float t = 0.0f;
-                        SC_HIP(hipEventElapsedTime(&t, ev[2 + 2 * sl], ev[3 + 2 * sl]));
-                        if (r > 0)
-                        {
-                            keep(tm[sl], t);
-                        }
+                        rec(1 + sl, r, 1);
                     }
                 }
             }
         }
+        SC_HIP(hipStreamSynchronize(stream));
+        auto best = [&](size_t slot)
+        {
+            float b = -1.0f;
+            for (int r = 1; r < TUNE_RUNS; ++r)
+            {
+                float t = 0.0f;
+                SC_HIP(hipEventElapsedTime(&t, evp(slot, r, 0), evp(slot, r, 1)));
+                b = b < 0.0f ? t : std::min(b, t);
+            }
+            return b;
+        };
+        float const solt = best(0);
+        std::vector<float> tm(nslot, -1.0f);
+        for (size_t sl = 0; sl < nslot; ++sl)
+        {
+            if (evp(1 + sl, 0, 0) != nullptr)
+            {
+                tm[sl] = best(1 + sl);
+            }
+        }
         for (auto & x : ev)
         {
-            (void)hipEventDestroy(x);
+            if (x != nullptr)
+            {
+                (void)hipEventDestroy(x);
+            }
         }
         if (std::getenv("SC_EULER_TUNE_LOG") != nullptr)
         {
