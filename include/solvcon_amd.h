@@ -139,10 +139,10 @@ int sceuler_counts(void * h, int64_t * out);
  * step from a captured hipGraph, 0 (default) to launch its kernels one by
  * one; read-only 4, 1 once the first substep has tuned the kernel forms
  * (else -1); read-only 100 + t, the form of the cells of type t: V for the
- * fused kernel in variant V (+ 100 when it reads so0t), 10000 + 100 A + B
- * for the split pair in variants A and B (a variant is the minimum
- * wavefronts per SIMD plus 10 when LDS-staged), 0 when the mesh has no such
- * cells. */
+ * fused kernel in variant V (+ 100 when it reads so0t, + 1000 when it
+ * prefetches its sub-face geometry into LDS), 10000 + 100 A + B for the
+ * split pair in variants A and B (a variant is the minimum wavefronts per
+ * SIMD plus 10 when LDS-staged), 0 when the mesh has no such cells. */
 double sceuler_get_param(void * h, int which);
 int sceuler_set_param(void * h, int which, double v);
 /* info[4]: rows, elements per row, ghost rows, rank of the table. */
