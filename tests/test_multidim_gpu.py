@@ -242,9 +242,16 @@ def test_march_every_kernel_form(orc, monkeypatch, name, form):
     defn = _defs()[name]
     g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
     _riemann(ge, oe, o, seed=4)
+    # Every boundary kind, so each variant's boundary code runs: slip walls
+    # on every third face, an inlet on one, non-reflective elsewhere.
     faces = list(o.bndfcs[:, 0])
     ge.add_nonrefl(faces)
     oe.add_bc(orc.BC_NONREFL, faces)
+    ge.add_slipwall(faces[1::3])
+    oe.add_bc(orc.BC_SLIPWALL, faces[1::3])
+    val = [1.05] + [0.2, 0.0, 0.0][:ge.ndim] + [1.0, GAMMA]
+    ge.add_inlet(faces[:1], value=val)
+    oe.add_bc(orc.BC_INLET, faces[:1], val)
     ge.bc_soln()
     ge.bc_dsoln()
     oe.call('bc_soln')
