@@ -91,6 +91,7 @@ enum Mode : int
     M_WAVES = M_W2 | M_W3 | M_W4,
     M_STAGE = 512, // stage the block's own rows in LDS for neighbour reads
     M_GDMA = 1024, // prefetch the sub-face geometry into LDS (LDS DMA)
+    M_STNEXT = 4096, // also write so0t of the level being written (chained so0t)
     M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
 };
 
@@ -211,6 +212,7 @@ struct KP
     double * s0n;
     double * s1n;
     double * st;
+    double * stn; // so0t of the level being written (chained so0t)
     double const * gam;
     double * cflo;
     double * cflc;
@@ -574,6 +576,20 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                             p.s1n[(j) * (NEQ * ND) + k * ND + d] = o1[k][d];
                         }
                     }
+                    if constexpr ((MODE & M_STNEXT) != 0)
+                    {
+                        // so0t of the ghost row for the level being
+                        // written (calc_solt_impl, euler_march.cpp:161-191).
+                        JacS<ND> gj;
+                        gj.set(p.gam[j], o0);
+                        double gst[NEQ];
+                        solt_of<ND>(gj, o1, gst);
+#pragma unroll
+                        for (int k = 0; k < NEQ; ++k)
+                        {
+                            p.stn[(j) * NEQ + k] = gst[k];
+                        }
+                    }
                 }
             }
         }
@@ -883,6 +899,30 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             for (int d = 0; d < ND; ++d)
             {
                 p.s1n[(row) * (NEQ * ND) + k * ND + d] = out[d];
+            }
+        }
+        if constexpr ((MODE & M_STNEXT) != 0)
+        {
+            // so0t of this row for the level being written (calc_solt_impl,
+            // euler_march.cpp:161-191), from the values just stored.
+            double s1w[NEQ][ND];
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                {
+                    s1w[k][d] = p.s1n[(row) * (NEQ * ND) + k * ND + d];
+                }
+            }
+            JacS<ND> rj;
+            rj.set(p.gam[row], u0);
+            double rst[NEQ];
+            solt_of<ND>(rj, s1w, rst);
+#pragma unroll
+            for (int k = 0; k < NEQ; ++k)
+            {
+                p.stn[(row) * NEQ + k] = rst[k];
             }
         }
     }
@@ -1791,7 +1831,8 @@ struct DEuler
     };
     std::vector<Group> groups;
     DevBuf<int32_t> int_of_ref, ref_of_int;
-    DevBuf<double> cen, s0[2], s1[2], st, stm, gam, cflo, cflc;
+    // st[b] holds so0t of the level in s0[b], s1[b].
+    DevBuf<double> cen, s0[2], s1[2], st[2], stm, gam, cflo, cflc;
     int cur = 0;
 
     std::vector<EulerBoundary> bcs;
@@ -1883,7 +1924,8 @@ struct DEuler
         p.s1c = s1[cur].ptr;
         p.s0n = s0[1 - cur].ptr;
         p.s1n = s1[1 - cur].ptr;
-        p.st = st.ptr;
+        p.st = st[cur].ptr;
+        p.stn = st[1 - cur].ptr;
         p.gam = gam.ptr;
         p.cflo = cflo.ptr;
         p.cflc = cflc.ptr;
@@ -2190,60 +2232,134 @@ struct DEuler
     static constexpr int M_SPLIT1 = M_SOLN | M_BC | M_CFL | M_STMEM;
     static constexpr int M_SPLIT2 = M_DSOLN | M_STMEM;
 
+    // Chained so0t.  When a kept form reads so0t from memory, the kernels
+    // that write the next level also write its so0t (body rows where so1n
+    // is written, ghost rows where the boundary conditions write them) into
+    // st[1 - cur], so the next substep finds it in st[cur] without a
+    // calc_solt pass.  Ghost rows of boundary faces without a condition are
+    // never written during a march: solt_both() computes so0t of both levels
+    // when a chain starts, and those rows stay valid.  chain_ready is false
+    // until then, and every march starts a new chain.
+    bool chain_ready = false;
+
+    bool chain_forms() const
+    {
+        for (auto const & f : gform)
+        {
+            if (needs_solt(f))
+            {
+                return true;
+            }
+        }
+        return false;
+    }
+
+    void solt_both()
+    {
+        calc_solt();
+        if (R == 0)
+        {
+            return;
+        }
+        KP q = params();
+        q.s0c = s0[1 - cur].ptr;
+        q.s1c = s1[1 - cur].ptr;
+        q.st = st[1 - cur].ptr;
+        if (ndim == 2)
+        {
+            k_solt<2><<<grid_for(R, BLK), BLK, 0, stream>>>(q);
+        }
+        else
+        {
+            k_solt<3><<<grid_for(R, BLK), BLK, 0, stream>>>(q);
+        }
+        SC_HIP(hipGetLastError());
+    }
+
+    // Starts the chain eagerly (outside any graph capture) when it is due.
+    void start_chain()
+    {
+        if (tuned != 0 && !chain_ready && chain_forms())
+        {
+            solt_both();
+            chain_ready = true;
+        }
+    }
+
     void substep_fused()
     {
         if (tuned == 0)
         {
             tune_substep();
+            chain_ready = false;
             return;
+        }
+        bool const chain = chain_forms();
+        if (chain && !chain_ready)
+        {
+            solt_both();
+            chain_ready = true;
         }
         update();
         KP const p = params();
-        bool any = false;
-        for (auto const & f : gform)
-        {
-            any = any || needs_solt(f);
-        }
-        if (any)
-        {
-            calc_solt();
-        }
         for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            run_form(p, groups[gi], gform[gi]);
+            run_form(p, groups[gi], gform[gi], chain);
         }
+        chain_ready = chain;
     }
 
     // Whether a form reads so0t from memory, so calc_solt must run first.
     static bool reads_solt(int code) { return (code / 100) % 10 != 0; }
     static bool needs_solt(Form const & f) { return f.split != 0 || reads_solt(f.vf); }
 
-    void run_role(KP const & p, Group const & gr, int role, int code, bool warm = false)
+    void run_role(KP const & p, Group const & gr, int role, int code, bool warm = false, bool chain = false)
     {
         if (role == 0)
         {
-            run_group_v<M_FUSED>(p, gr, code, warm);
+            if (chain)
+            {
+                run_group_v<M_FUSED | M_STNEXT>(p, gr, code, warm);
+            }
+            else
+            {
+                run_group_v<M_FUSED>(p, gr, code, warm);
+            }
         }
         else if (role == 1)
         {
-            run_group_v<M_SPLIT1>(p, gr, code, warm);
+            if (chain)
+            {
+                run_group_v<M_SPLIT1 | M_STNEXT>(p, gr, code, warm);
+            }
+            else
+            {
+                run_group_v<M_SPLIT1>(p, gr, code, warm);
+            }
         }
         else
         {
-            run_group_v<M_SPLIT2>(p, gr, code, warm);
+            if (chain)
+            {
+                run_group_v<M_SPLIT2 | M_STNEXT>(p, gr, code, warm);
+            }
+            else
+            {
+                run_group_v<M_SPLIT2>(p, gr, code, warm);
+            }
         }
     }
 
-    void run_form(KP const & p, Group const & gr, Form const & f)
+    void run_form(KP const & p, Group const & gr, Form const & f, bool chain = false)
     {
         if (f.split != 0)
         {
-            run_role(p, gr, 1, f.v1);
-            run_role(p, gr, 2, f.v2);
+            run_role(p, gr, 1, f.v1, false, chain);
+            run_role(p, gr, 2, f.v2, false, chain);
         }
         else
         {
-            run_role(p, gr, 0, f.vf);
+            run_role(p, gr, 0, f.vf, false, chain);
         }
     }
 
@@ -2270,7 +2386,7 @@ struct DEuler
                     int const c = role_variant(ndim, role, v);
                     if (c != 0 && (!has_gdma(c) || gdma_ok(groups[gi], c)))
                     {
-                        run_role(p, groups[gi], role, c, true);
+                        run_role(p, groups[gi], role, c, true, true);
                     }
                 }
             }
@@ -2281,9 +2397,12 @@ struct DEuler
         // variants a few percent apart would swap places from run to run.
         constexpr int TUNE_RUNS = 4;
         // Event pairs by (slot, run): slot 0 times calc_solt, slot 1 + sl
-        // the variant of timing slot sl.  Everything is enqueued first and
-        // the stream synchronised once.
-        std::vector<hipEvent_t> ev(2 * TUNE_RUNS * (1 + nslot), nullptr);
+        // the variant of timing slot sl with the chained so0t write (as
+        // every kernel of a substep whose forms read so0t runs), slot
+        // 1 + nslot + sl a fused variant that recomputes so0t without it (as
+        // it runs when no form reads so0t).  Everything is enqueued first
+        // and the stream synchronised once.
+        std::vector<hipEvent_t> ev(2 * TUNE_RUNS * (1 + 2 * nslot), nullptr);
         auto evp = [&](size_t slot, int r, int b) -> hipEvent_t & { return ev[(slot * TUNE_RUNS + r) * 2 + b]; };
         auto rec = [&](size_t slot, int r, int b)
         {
@@ -2312,11 +2431,19 @@ struct DEuler
                         continue;
                     }
                     size_t const sl = (gi * 3 + role) * NVAR + v;
-                    for (int r = 0; r < TUNE_RUNS; ++r)
+                    for (int chain = 0; chain < 2; ++chain)
                     {
-                        rec(1 + sl, r, 0);
-                        run_role(p, groups[gi], role, c);
-                        rec(1 + sl, r, 1);
+                        if (chain == 0 && (role != 0 || reads_solt(c)))
+                        {
+                            continue;
+                        }
+                        size_t const slot = chain != 0 ? 1 + sl : 1 + nslot + sl;
+                        for (int r = 0; r < TUNE_RUNS; ++r)
+                        {
+                            rec(slot, r, 0);
+                            run_role(p, groups[gi], role, c, false, chain != 0);
+                            rec(slot, r, 1);
+                        }
                     }
                 }
             }
@@ -2334,12 +2461,17 @@ struct DEuler
             return b;
         };
         float const solt = best(0);
-        std::vector<float> tm(nslot, -1.0f);
+        std::vector<float> tm(nslot, -1.0f); // chained
+        std::vector<float> tp(nslot, -1.0f); // plain fused that recomputes so0t
         for (size_t sl = 0; sl < nslot; ++sl)
         {
             if (evp(1 + sl, 0, 0) != nullptr)
             {
                 tm[sl] = best(1 + sl);
+            }
+            if (evp(1 + nslot + sl, 0, 0) != nullptr)
+            {
+                tp[sl] = best(1 + nslot + sl);
             }
         }
         for (auto & x : ev)
@@ -2351,7 +2483,7 @@ struct DEuler
         }
         if (std::getenv("SC_EULER_TUNE_LOG") != nullptr)
         {
-            std::fprintf(stderr, "EulerCore tuning: calc_solt %.4f ms\n", solt);
+            std::fprintf(stderr, "EulerCore tuning (ms; chained so0t write, plain in brackets): calc_solt %.4f\n", solt);
             for (size_t gi = 0; gi < ng; ++gi)
             {
                 for (int role = 0; role < 3; ++role)
@@ -2361,82 +2493,84 @@ struct DEuler
                                  role == 0 ? "fused " : role == 1 ? "split1" : "split2");
                     for (int v = 0; v < NVAR; ++v)
                     {
-                        float const t = tm[(gi * 3 + role) * NVAR + v];
+                        size_t const sl = (gi * 3 + role) * NVAR + v;
                         int const c = role_variant(ndim, role, v);
-                        if (t >= 0.0f)
+                        if (tm[sl] >= 0.0f)
                         {
-                            std::fprintf(stderr, "  w%d%s%s%s %.4f ms", c % 10, (c / 10) % 10 != 0 ? "+lds" : "",
-                                         (c / 100) % 10 != 0 ? "+so0t" : "", (c / 1000) % 10 != 0 ? "+gdma" : "", t);
+                            std::fprintf(stderr, "  w%d%s%s%s %.4f", c % 10, (c / 10) % 10 != 0 ? "+lds" : "",
+                                         (c / 100) % 10 != 0 ? "+so0t" : "", (c / 1000) % 10 != 0 ? "+gdma" : "", tm[sl]);
+                            if (tp[sl] >= 0.0f)
+                            {
+                                std::fprintf(stderr, " [%.4f]", tp[sl]);
+                            }
                         }
                     }
                     std::fprintf(stderr, "\n");
                 }
             }
         }
-        // Per group: the best fused variant that recomputes so0t (fast),
-        // the best one that reads it (fastt), the best split pair.  Either
-        // no group reads so0t, or calc_solt runs once and every group takes
-        // its best form of all.
-        std::vector<Form> fast(ng), fastt(ng), split(ng);
-        std::vector<double> tf(ng, -1.0), tft(ng, -1.0), ts(ng, 0.0);
+        // Two configurations.  Plain: every group runs its fastest fused
+        // variant that recomputes so0t, without the chained write.  Chained:
+        // every group takes its fastest form of all, each timed with the
+        // chained write (so0t is then chained from the kernels that write
+        // each level, see substep_fused, and no calc_solt runs per substep).
+        std::vector<Form> fast(ng), pick(ng);
+        double no_solt = 0.0;
+        double with_solt = 0.0;
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            double best[3] = {-1.0, -1.0, -1.0};
-            int bc[3] = {1, 1, 1};
+            double tf = -1.0;
+            double tc = -1.0;
+            double best1 = -1.0;
+            double best2 = -1.0;
+            int bc1 = 1;
+            int bc2 = 1;
             for (int role = 0; role < 3; ++role)
             {
                 for (int v = 0; v < NVAR; ++v)
                 {
-                    float const t = tm[(gi * 3 + role) * NVAR + v];
+                    size_t const sl = (gi * 3 + role) * NVAR + v;
                     int const c = role_variant(ndim, role, v);
-                    if (t < 0.0f)
+                    if (role == 0 && tp[sl] >= 0.0f && (tf < 0.0 || tp[sl] < tf))
+                    {
+                        tf = tp[sl];
+                        fast[gi].vf = c;
+                    }
+                    if (tm[sl] < 0.0f)
                     {
                         continue;
                     }
-                    if (role == 0)
+                    if (role == 0 && (tc < 0.0 || tm[sl] < tc))
                     {
-                        double & b = reads_solt(c) ? tft[gi] : tf[gi];
-                        if (b < 0.0 || t < b)
-                        {
-                            b = t;
-                            (reads_solt(c) ? fastt[gi] : fast[gi]).vf = c;
-                        }
+                        tc = tm[sl];
+                        pick[gi] = Form{};
+                        pick[gi].vf = c;
                     }
-                    else if (best[role] < 0.0 || t < best[role])
+                    else if (role == 1 && (best1 < 0.0 || tm[sl] < best1))
                     {
-                        best[role] = t;
-                        bc[role] = c;
+                        best1 = tm[sl];
+                        bc1 = c;
+                    }
+                    else if (role == 2 && (best2 < 0.0 || tm[sl] < best2))
+                    {
+                        best2 = tm[sl];
+                        bc2 = c;
                     }
                 }
             }
-            split[gi].split = 1;
-            split[gi].v1 = bc[1];
-            split[gi].v2 = bc[2];
-            ts[gi] = best[1] + best[2];
+            if (best1 >= 0.0 && best2 >= 0.0 && (tc < 0.0 || best1 + best2 < tc))
+            {
+                tc = best1 + best2;
+                pick[gi] = Form{1, 1, bc1, bc2};
+            }
+            no_solt += tf;
+            with_solt += tc;
         }
-        double no_solt = 0.0;
-        double with_solt = solt;
-        std::vector<Form> pick(ng);
+        bool const chained = with_solt < no_solt && std::any_of(pick.begin(), pick.end(), [](Form const & f)
+                                                                { return needs_solt(f); });
         for (size_t gi = 0; gi < ng; ++gi)
         {
-            no_solt += tf[gi];
-            double b = tf[gi];
-            pick[gi] = fast[gi];
-            if (tft[gi] >= 0.0 && tft[gi] < b)
-            {
-                b = tft[gi];
-                pick[gi] = fastt[gi];
-            }
-            if (ts[gi] < b)
-            {
-                b = ts[gi];
-                pick[gi] = split[gi];
-            }
-            with_solt += b;
-        }
-        for (size_t gi = 0; gi < ng; ++gi)
-        {
-            gform[gi] = with_solt < no_solt ? pick[gi] : fast[gi];
+            gform[gi] = chained ? pick[gi] : fast[gi];
         }
         tuned = 1;
     }
@@ -2501,10 +2635,15 @@ struct DEuler
     void march(int64_t steps)
     {
         int64_t sub = 2 * steps;
+        chain_ready = false;
         if (sub > 0 && tuned == 0)
         {
             substep_fused();
             --sub;
+        }
+        if (sub > 0)
+        {
+            start_chain();
         }
         if (use_graph != 0 && sub >= 2)
         {
@@ -2543,6 +2682,7 @@ struct DEuler
 
     void march_substep()
     {
+        chain_ready = false;
         substep_fused();
         calc_solt();
     }
@@ -2828,12 +2968,13 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         s0[b].alloc(static_cast<int64_t>(neq) * R);
         s1[b].alloc(static_cast<int64_t>(neq) * ndim * R);
     }
-    st.alloc(static_cast<int64_t>(neq) * R);
+    st[0].alloc(static_cast<int64_t>(neq) * R);
+    st[1].alloc(static_cast<int64_t>(neq) * R);
     stm.alloc(static_cast<int64_t>(neq) * R);
     gam.alloc(R);
     cflo.alloc(R);
     cflc.alloc(R);
-    for (auto * b : {&s0[0], &s0[1], &s1[0], &s1[1], &st, &stm, &gam, &cflo, &cflc, &cen})
+    for (auto * b : {&s0[0], &s0[1], &s1[0], &s1[1], &st[0], &st[1], &stm, &gam, &cflo, &cflc, &cen})
     {
         if (b->count)
         {
@@ -3240,7 +3381,7 @@ void DEuler::copy(int id, double * host, int64_t count, bool down)
         {
         case EA_SO0C: soa = s0[cur].ptr; break;
         case EA_SO0N: soa = s0[1 - cur].ptr; break;
-        case EA_SO0T: soa = st.ptr; break;
+        case EA_SO0T: soa = st[cur].ptr; break;
         case EA_STM: soa = stm.ptr; break;
         case EA_SO1C: soa = s1[cur].ptr; break;
         case EA_SO1N: soa = s1[1 - cur].ptr; break;
@@ -3590,11 +3731,14 @@ int sceuler_march_timed(void * h, int64_t steps, double * ms_substep, double * m
             }
             SC_HIP(hipEventRecord(ev[2 * nsub], e.stream));
             int64_t counted = 0;
+            e.chain_ready = false;
+            e.start_chain();
             for (int64_t it = 0; it < nsub; ++it)
             {
                 if (e.tuned == 0)
                 {
                     e.substep_fused(); // the tuning substep, not counted
+                    e.start_chain();
                     SC_HIP(hipEventRecord(ev[2 * it], e.stream));
                     SC_HIP(hipEventRecord(ev[2 * it + 1], e.stream));
                     continue;
