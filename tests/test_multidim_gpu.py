@@ -532,3 +532,49 @@ def test_graph_replay_equals_eager():
         for t in ('so0n', 'so1n', 'so0c', 'so1c', 'so0t', 'cflo', 'cflc'):
             assert np.array_equal(getattr(a, t).ndarray,
                                   getattr(b, t).ndarray), (k, t)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('form', ['6:f2st,7:s1s/2s,5:s1s/1,8:s1s/2', 'f2'])
+def test_chained_so0t_across_calls(orc, monkeypatch, form):
+    # With forms that read so0t, each level's kernels write the next
+    # level's so0t.  The chain restarts at every march call: state uploads,
+    # new boundary conditions and parameter changes between calls, single
+    # substeps, and boundary faces left without a condition (their ghost
+    # rows are never written during a march) all keep the oracle's values,
+    # so0t included.
+    print('case: chained so0t across calls, forms', form, flush=True)
+    monkeypatch.setenv('SC_EULER_FORM', form)
+    defn = _defs()['mixed3d']
+    g, o, ge, oe = _euler_pair(orc, defn, dt=2e-3)
+    _riemann(ge, oe, o, seed=5)
+    faces = list(o.bndfcs[:, 0])
+    ge.add_nonrefl(faces[0::3])
+    oe.add_bc(orc.BC_NONREFL, faces[0::3])
+    ge.add_slipwall(faces[1::3])
+    oe.add_bc(orc.BC_SLIPWALL, faces[1::3])
+    ge.bc_soln()
+    ge.bc_dsoln()
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    ge.march(2)
+    oe.march(2)
+    pair.same_euler(ge, oe)
+    n = ge.ngstcell
+    bump = np.linspace(0.0, 0.01, ge.ncell)
+    ge.so0n.ndarray[n:, 0] += bump
+    oe.so0n[n:, 0] += bump
+    ge.march(1)
+    oe.march(1)
+    pair.same_euler(ge, oe)
+    # Faces 2, 8, 14, ... had no condition so far.
+    ge.add_slipwall(faces[2::6])
+    oe.add_bc(orc.BC_SLIPWALL, faces[2::6])
+    ge.sigma0 = 2.0
+    oe.sigma0 = 2.0
+    ge.march(3)
+    oe.march(3)
+    pair.same_euler(ge, oe)
+    ge.march_substep()
+    oe.call('march_substep')
+    pair.same_euler(ge, oe)
