@@ -348,6 +348,7 @@ def run_batched(args):
     t3 = time.perf_counter()
     dev_ms = b.march()
     t4 = time.perf_counter()
+    done = b.done_ms()
     wall = t4 - t3
     log('bench: batched march %.3f s wall (%.1f ms device), %.4g '
         'cell-updates/s, %.1f solves/s' %
@@ -388,6 +389,20 @@ def run_batched(args):
         'cell-update; whole batch serially %.1f s march + %.1f s set-up; '
         'all equal: %s' % (len(pick), coef[0], coef[1], plain_march,
                            plain_setup, bool(tm[:, 4].all())))
+    # Latency of each solve, all submitted at the march's start: batched,
+    # the device time until the instance's last step ran; serially, the
+    # fitted march times of the instances before it and its own, in
+    # instance order.
+    lat = done / 1e3
+    slat = np.cumsum(coef[0] * steps + coef[1] * b.ncells * steps)
+    pct = {'p50_s': float(np.percentile(lat, 50)),
+           'p99_s': float(np.percentile(lat, 99)),
+           'serial_p50_s_est': float(np.percentile(slat, 50)),
+           'serial_p99_s_est': float(np.percentile(slat, 99))}
+    log('bench: solve latency p50/p99 %.2f/%.2f s batched, %.2f/%.2f s '
+        'serially (est.)' % (pct['p50_s'], pct['p99_s'],
+                             pct['serial_p50_s_est'],
+                             pct['serial_p99_s_est']))
     out = {
         'metric': 'batched solves/s (FP64 Euler CESE, 2D triangles)',
         'value': round(b.ninst / wall, 2),
@@ -410,7 +425,9 @@ def run_batched(args):
             'plain_serial_setup_s_est': round(plain_setup, 2),
             'plain_sample': int(len(pick)),
             'sample_equal_bitwise': bool(tm[:, 4].all()),
+            'latency_s': {k: round(v, 3) for k, v in pct.items()},
         },
+        'p99_s': round(pct['p99_s'], 3),
         'vs_plain': round(plain_march / wall, 3),
         'vs_plain_end_to_end': round((plain_march + plain_setup)
                                      / (wall + t2 - t1), 3),

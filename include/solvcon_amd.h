@@ -182,6 +182,11 @@ int sceuler_batch_cfl_dt(void * h, double cfl, double * dt_out);
 /* March every instance its own step count; *ms_total (if not null)
  * receives the device time of the march. */
 int sceuler_march_batch(void * h, double * ms_total);
+/* Completion time of every instance (by instance id) in the last timed
+ * sceuler_march_batch: milliseconds from the march's start until the
+ * instance's last step ran, on the device clock (n == ninst; no reference
+ * counterpart). */
+int sceuler_batch_done_ms(void * h, double * out, int64_t n);
 /* march(steps) with HIP-event timing: the average duration of one
  * substep's fused kernels and the whole march, in milliseconds (no
  * reference counterpart; used by the benchmark). */

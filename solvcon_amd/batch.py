@@ -121,6 +121,11 @@ class BatchedEuler(object):
         """March every instance its own step count; returns device ms."""
         return self.core.march_batch()
 
+    def done_ms(self):
+        """Per instance, device ms from the start of the last march until
+        its last step ran (instances with fewer steps finish earlier)."""
+        return self.core.batch_done_ms()
+
     @property
     def cell_updates(self):
         return int((self.ncells * self.steps).sum())

@@ -1849,6 +1849,13 @@ struct DEuler
     std::vector<std::vector<int64_t>> bcum;
     DevBuf<int32_t> brank_ref, binst;
     DevBuf<double> bhdt, bqdt;
+    // Completion times of a timed batched march: an event is recorded on
+    // the stream when the ranks [first, second) have run their last step,
+    // and bdone_ms[rank] receives its time from the march's start.
+    bool btime = false;
+    std::vector<std::pair<int64_t, int64_t>> bdone_rng;
+    std::vector<hipEvent_t> bdone_ev;
+    std::vector<double> bdone_ms;
 
     DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch = 0, int32_t const * inst_of_cell = nullptr,
            int64_t const * steps = nullptr);
@@ -2099,12 +2106,25 @@ struct DEuler
         int64_t nact = rb;
         hipGraphExec_t g = nullptr;
         int64_t gact = -1;
+        auto done = [&](int64_t lo, int64_t hi)
+        {
+            if (btime && lo < hi)
+            {
+                hipEvent_t ev = nullptr;
+                SC_HIP(hipEventCreate(&ev));
+                bdone_ev.push_back(ev);
+                bdone_rng.emplace_back(lo, hi);
+                SC_HIP(hipEventRecord(ev, stream));
+            }
+        };
         for (int64_t step = 0; step < maxs; ++step)
         {
+            int64_t const was = nact;
             while (nact > ra && bsteps[nact - 1] <= step)
             {
                 --nact;
             }
+            done(nact, was);
             if (use_graph != 0)
             {
                 if (g == nullptr || gact != nact)
@@ -2130,6 +2150,7 @@ struct DEuler
                 run_cells_batch(ra, nact);
             }
         }
+        done(ra, nact);
     }
 
     void march_batch()
@@ -3499,6 +3520,18 @@ int sceuler_march_batch(void * h, double * ms_total)
             }
             hipEvent_t a = nullptr;
             hipEvent_t b = nullptr;
+            auto drop_done = [&]()
+            {
+                for (auto ev : e.bdone_ev)
+                {
+                    (void)hipEventDestroy(ev);
+                }
+                e.bdone_ev.clear();
+                e.bdone_rng.clear();
+            };
+            drop_done();
+            e.bdone_ms.assign(static_cast<size_t>(e.ninst), -1.0);
+            e.btime = ms_total != nullptr;
             if (ms_total != nullptr)
             {
                 SC_HIP(hipEventCreate(&a));
@@ -3506,6 +3539,7 @@ int sceuler_march_batch(void * h, double * ms_total)
                 SC_HIP(hipEventRecord(a, e.stream));
             }
             e.march_batch();
+            e.btime = false;
             if (ms_total != nullptr)
             {
                 SC_HIP(hipEventRecord(b, e.stream));
@@ -3516,8 +3550,43 @@ int sceuler_march_batch(void * h, double * ms_total)
                 float ms = 0.0f;
                 SC_HIP(hipEventElapsedTime(&ms, a, b));
                 *ms_total = ms;
+                for (size_t i = 0; i < e.bdone_ev.size(); ++i)
+                {
+                    float t = 0.0f;
+                    SC_HIP(hipEventElapsedTime(&t, a, e.bdone_ev[i]));
+                    for (int64_t r = e.bdone_rng[i].first; r < e.bdone_rng[i].second; ++r)
+                    {
+                        e.bdone_ms[static_cast<size_t>(r)] = t;
+                    }
+                }
+                drop_done();
                 (void)hipEventDestroy(a);
                 (void)hipEventDestroy(b);
+            }
+        });
+}
+
+int sceuler_batch_done_ms(void * h, double * out, int64_t n)
+{
+    return guard(
+        [&]()
+        {
+            DEuler const & e = *static_cast<DEuler const *>(h);
+            if (e.ninst == 0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore: not a batched solver");
+            }
+            if (n != e.ninst)
+            {
+                scamd::fail(scamd::SC_EVALUE, "EulerCore batch: completion times need one slot per instance");
+            }
+            if (e.bdone_ms.size() != static_cast<size_t>(e.ninst) || e.bdone_ms[0] < 0.0)
+            {
+                scamd::fail(scamd::SC_ERUNTIME, "EulerCore batch: no timed march_batch yet");
+            }
+            for (int64_t k = 0; k < n; ++k)
+            {
+                out[k] = e.bdone_ms[static_cast<size_t>(e.brank_of[static_cast<size_t>(k)])];
             }
         });
 }

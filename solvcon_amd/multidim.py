@@ -42,6 +42,7 @@ L.register({
     'sceuler_batch_set_dt': (L.c_int, [L.c_vp, L.P_dbl]),
     'sceuler_batch_cfl_dt': (L.c_int, [L.c_vp, L.c_dbl, L.P_dbl]),
     'sceuler_march_batch': (L.c_int, [L.c_vp, L.P_dbl]),
+    'sceuler_batch_done_ms': (L.c_int, [L.c_vp, L.P_dbl, L.c_i64]),
     'sceuler_cfl_dt': (L.c_int, [L.c_vp, L.c_dbl, L.P_dbl]),
     'scge_build': (L.c_int, [L.c_vp, L.P_dbl, L.c_i64, L.c_i64, L.c_i64,
                              L.c_i32, L.c_dbl, L.P_i32, L.P_dbl]),
@@ -354,6 +355,15 @@ class EulerCore(object):
         finally:
             self._after()
         return ms.value
+
+    def batch_done_ms(self):
+        """Per instance (by instance id), the milliseconds from the start
+        of the last ``march_batch`` until that instance's last step ran, on
+        the device clock."""
+        out = np.zeros(self._ninst, dtype=np.float64)
+        L.check(L.lib().sceuler_batch_done_ms(
+            self._h, out.ctypes.data_as(L.P_dbl), out.size))
+        return out
 
     # -- boundary conditions ------------------------------------------------------
 

@@ -163,6 +163,40 @@ def test_gpu_batch_guards():
         ec.batch_set_dt([0.1])
     with pytest.raises(ValueError):
         BatchedEuler([d], [st], [0])
+    with pytest.raises(RuntimeError):
+        ec.batch_done_ms()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('chunk', [0, 150])
+def test_gpu_batch_completion_times(chunk):
+    # Each instance's completion time is when its last step ran: within
+    # the march, and (one chunk) ordered by step count, equal for equal
+    # counts, the longest at the march's end.
+    print('case: batch completion times, chunk', chunk, flush=True)
+    from solvcon_amd.batch import BatchedEuler
+    from solvcon_amd.gen import tri_mesh_def
+    defs = [tri_mesh_def(m, seed=m) for m in (4, 6, 3, 5, 7, 4, 6, 5)]
+    steps = [4, 7, 2, 7, 5, 1, 6, 3]
+    states = []
+    for d in defs:
+        st = np.zeros((len(d[2]), 4))
+        st[:, 0] = 1.0
+        st[:, 3] = 2.5
+        states.append(st)
+    b = BatchedEuler(defs, states, steps, dts=[0.01] * 8, chunk_cells=chunk)
+    total = b.march()
+    done = b.done_ms()
+    assert done.shape == (8,)
+    assert (done > 0).all() and (done <= total * (1 + 1e-6)).all()
+    if chunk == 0:
+        for i in range(8):
+            for j in range(8):
+                if steps[i] < steps[j]:
+                    assert done[i] <= done[j], (i, j)
+                elif steps[i] == steps[j]:
+                    assert done[i] == done[j], (i, j)
+        assert done[1] >= 0.5 * total
 
 
 @pytest.mark.gpu
