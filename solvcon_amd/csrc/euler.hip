@@ -1863,6 +1863,10 @@ struct DEuler
     {
         (void)hipStreamSynchronize(stream);
         drop_graphs();
+        for (auto ev : bdone_ev) // left by a timed batched march that failed
+        {
+            (void)hipEventDestroy(ev);
+        }
     }
 
     void sync() { SC_HIP(hipStreamSynchronize(stream)); }
