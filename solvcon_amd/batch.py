@@ -38,15 +38,22 @@ def union_def(defs):
     noff = np.concatenate([[0], np.cumsum(nnode)])
     ndcrd = np.concatenate([np.asarray(d[1], dtype=np.float64) for d in defs])
     cltpn = np.concatenate([np.asarray(d[2], dtype=np.int32) for d in defs])
+    if noff[-1] > np.iinfo(np.int32).max:
+        raise ValueError('union_def: more nodes than int32 indexes')
     clnds = np.zeros((coff[-1], width), dtype=np.int32)
     for k, d in enumerate(defs):
+        # The node ids of every cell (its first clnds[:, 0] entries) are
+        # shifted by the instance's node offset.  When every cell fills
+        # its row, the shift is one add fused with the copy.
         c = np.asarray(d[3], dtype=np.int32)
         blk = clnds[coff[k]:coff[k + 1], :c.shape[1]]
-        blk[...] = c
-        cnt = c[:, 0]
-        cols = np.arange(1, c.shape[1])[None, :]
-        mask = cols <= cnt[:, None]
-        blk[:, 1:][mask] += noff[k]
+        blk[:, 0] = c[:, 0]
+        if (c[:, 0] == c.shape[1] - 1).all():
+            np.add(c[:, 1:], np.int32(noff[k]), out=blk[:, 1:])
+        else:
+            blk[:, 1:] = c[:, 1:]
+            mask = np.arange(1, c.shape[1])[None, :] <= c[:, :1]
+            blk[:, 1:][mask] += np.int32(noff[k])
     inst = np.repeat(np.arange(len(defs), dtype=np.int32), ncell)
     return (ndim, ndcrd, cltpn, clnds), inst, coff, noff
 
