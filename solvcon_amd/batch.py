@@ -47,8 +47,15 @@ def union_def(defs):
         # its row, the shift is one add fused with the copy.
         c = np.asarray(d[3], dtype=np.int32)
         blk = clnds[coff[k]:coff[k + 1], :c.shape[1]]
+        full = (c[:, 0] == c.shape[1] - 1).all()
+        if full and c.shape[1] == width:
+            # Contiguous rows: shift every entry, then restore the counts.
+            flat = clnds[coff[k]:coff[k + 1]].reshape(-1)
+            np.add(c.reshape(-1), np.int32(noff[k]), out=flat)
+            flat[0::width] = c[:, 0]
+            continue
         blk[:, 0] = c[:, 0]
-        if (c[:, 0] == c.shape[1] - 1).all():
+        if full:
             np.add(c[:, 1:], np.int32(noff[k]), out=blk[:, 1:])
         else:
             blk[:, 1:] = c[:, 1:]
