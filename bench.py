@@ -274,15 +274,10 @@ def run_mesh(args, which):
 def run_1d(args):
     """Sod tube, 2001 points, 600 steps of march_alpha2 per run
     (cell-updates per run = 600 x (2001 - 4))."""
-    from solvcon_amd.onedim import ShockTube
+    from solvcon_amd.gen import sod_tube
     from solvcon_amd import _lib
     ncoord, steps = 2001, 600
-    st = ShockTube()
-    st.build_constant(gamma=1.4, pressure1=1.0, density1=1.0,
-                      pressure5=0.1, density5=0.125)
-    st.build_numerical(xmin=-1, xmax=1, ncoord=ncoord,
-                       time_increment=0.4 / steps)
-    base = st.svr
+    base = sod_tube(ncoord=ncoord, time_increment=0.4 / steps)
     runs = []
     for _ in range(args.warmup + args.steps):
         s = base.clone()

@@ -5,10 +5,12 @@ so the FP64 arithmetic of every kernel follows source order exactly; packed
 FP32 instructions are disabled.
 
 The device code of the multidimensional march (euler.hip) is also compiled
-with -fno-signed-zeros -ffinite-math-only.  For finite states this only
-lets the compiler drop additions of +0 and products with the Euler
-Jacobian's structural zeros.  Every value stays equal to the reference's;
-only the sign of an exactly zero result may differ.
+with -fno-signed-zeros.  It only lets the compiler drop additions of +0
+(accumulators that start at zero): every value stays equal to the
+reference's, infinities and NaNs included; only the sign of an exactly zero
+result may differ.  Finite-math assumptions are not used: a state that
+overflows propagates its infinities and NaNs exactly as the reference's
+IEEE arithmetic does.
 """
 
 import os
@@ -38,8 +40,7 @@ def hipcc():
 
 
 PER_SOURCE = {
-    'euler.hip': ['-Xarch_device', '-fno-signed-zeros',
-                  '-Xarch_device', '-ffinite-math-only'],
+    'euler.hip': ['-Xarch_device', '-fno-signed-zeros'],
 }
 
 

@@ -9,7 +9,9 @@ counter-clockwise triangles along a seeded diagonal; cells are numbered in
 
 import numpy as np
 
-__all__ = ['tri_mesh_def', 'riemann_box', 'build_static_mesh', 'cfl_dt']
+__all__ = ['tri_mesh_def', 'riemann_box', 'build_static_mesh', 'cfl_dt',
+           'sod_tube', 'mixed_mesh_def', 'blast_state', 'rect_mesh_def',
+           'edge_faces']
 
 
 def tri_mesh_def(m, seed=2, jitter=0.2):
@@ -274,3 +276,85 @@ def blast_state(centroids, gamma=1.4, r=0.1, center=(0.5, 0.5, 0.5),
     out[:, 0] = rho
     out[:, 4] = np.where(d < r, p_in, p_out) / (gamma - 1.0)
     return out
+
+
+SOD_LEFT = (1.0, 0.0, 1.0)
+SOD_RIGHT = (0.125, 0.0, 0.1)
+
+
+def sod_tube(ncoord=2001, time_increment=0.4 / 600, xmin=-1.0, xmax=1.0,
+             left=SOD_LEFT, right=SOD_RIGHT, gamma=1.4, split=0.0):
+    """A GPU Euler1DCore holding a shock tube ready to march.
+
+    ``ncoord`` uniform points on [xmin, xmax]; the primitive states
+    ``left`` / ``right`` (rho, u, p) meet at ``split`` (a point at the split
+    takes the right state); gradients start at zero; the initial CFL
+    numbers are set by ``setup_march``."""
+    from .onedim import Euler1DCore
+    core = Euler1DCore(ncoord=ncoord, time_increment=time_increment)
+    x = np.linspace(xmin, xmax, num=ncoord)
+    prim = np.where((x < split)[:, None], np.asarray(left, float),
+                    np.asarray(right, float))
+    rho, u, p = prim[:, 0], prim[:, 1], prim[:, 2]
+    so0 = np.stack([rho, rho * u, p / (gamma - 1.0) + 0.5 * rho * u * u],
+                   axis=1)
+    core.coord[...] = x
+    core.gamma.fill(gamma)
+    core.so0[...] = so0
+    core.so1[...] = 0.0
+    core.cfl.fill(0.0)
+    core.setup_march()
+    return core
+
+
+def rect_mesh_def(nx, ny, extent=(4.0, 1.0), kind='quad', seed=0,
+                  jitter=0.0):
+    """(ndim, ndcrd, cltpn, clnds) of an nx-by-ny box grid on
+    [0, extent[0]] x [0, extent[1]].
+
+    ``kind`` 'quad' keeps one counter-clockwise quadrilateral per box;
+    'tri' splits each box into two counter-clockwise triangles along a
+    seeded diagonal.  Interior nodes move by up to ``jitter`` box widths
+    per axis.  Boxes are numbered row by row from the lower left."""
+    if kind not in ('quad', 'tri'):
+        raise ValueError("kind must be 'quad' or 'tri'")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    hx, hy = extent[0] / nx, extent[1] / ny
+    j, i = np.meshgrid(np.arange(ny + 1), np.arange(nx + 1), indexing='ij')
+    x, y = i * hx, j * hy
+    if jitter:
+        inner = (i > 0) & (i < nx) & (j > 0) & (j < ny)
+        x = np.where(inner, x + rng.uniform(-jitter, jitter, x.shape) * hx,
+                     x)
+        y = np.where(inner, y + rng.uniform(-jitter, jitter, y.shape) * hy,
+                     y)
+    ndcrd = np.stack([x.ravel(), y.ravel()], axis=1)
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing='ij')
+    a = (jj * (nx + 1) + ii).ravel()
+    b, c, d = a + 1, a + nx + 2, a + nx + 1
+    if kind == 'quad':
+        clnds = np.stack([np.full_like(a, 4), a, b, c, d], axis=1)
+        cltpn = np.full(a.size, 3, dtype=np.int32)
+    else:
+        up = rng.integers(0, 2, size=a.size).astype(bool)
+        t0 = np.where(up[:, None], np.stack([a, b, c], 1),
+                      np.stack([a, b, d], 1))
+        t1 = np.where(up[:, None], np.stack([a, c, d], 1),
+                      np.stack([b, c, d], 1))
+        tri = np.stack([t0, t1], axis=1).reshape(-1, 3)
+        clnds = np.concatenate([np.full((tri.shape[0], 1), 3), tri], axis=1)
+        cltpn = np.full(tri.shape[0], 4, dtype=np.int32)
+    return 2, ndcrd, cltpn, clnds.astype(np.int32)
+
+
+def edge_faces(bndfcs, fccnd, extent, tol=1e-9):
+    """Boundary faces of a rectangle [0, X] x [0, Y] split by edge:
+    (left, top, bottom, right), each in ascending face order.  ``bndfcs``
+    holds face indices into ``fccnd`` (body rows)."""
+    f = np.sort(np.asarray(bndfcs))
+    c = np.asarray(fccnd)[f]
+    left = f[np.abs(c[:, 0]) < tol]
+    right = f[np.abs(c[:, 0] - extent[0]) < tol]
+    bottom = f[np.abs(c[:, 1]) < tol]
+    top = f[np.abs(c[:, 1] - extent[1]) < tol]
+    return left, top, bottom, right

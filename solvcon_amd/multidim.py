@@ -1,12 +1,13 @@
 """Multidimensional Euler CESE solver on the GPU.
 
 Drop-in for solvcon's ``EulerCore`` and ``GradientElement`` bindings
-(cpp/solvcon/multidim/pymod/wrap_multidim.cpp:17-414) and for the
-``EulerField`` views (solvcon/multidim/euler/_field.py:16-148).  State and
-geometry live in HBM; every stage runs a HIP kernel of euler.hip.
+(cpp/solvcon/multidim/pymod/wrap_multidim.cpp:17-414).  State and geometry
+live in HBM; every stage runs a HIP kernel of euler.hip.  solvcon's Python
+``EulerField`` (solvcon/multidim/euler/_field.py) reads these arrays
+unchanged; ``EulerCore.reduce`` gives its mass and CFL range from one FP64
+device reduction.
 """
 
-import collections
 import ctypes
 
 import numpy as np
@@ -14,8 +15,7 @@ import numpy as np
 from . import _lib as L
 from .array import GhostArray
 
-__all__ = ['EulerCore', 'GradientElement', 'EulerField', 'RunHistory',
-           'RunRecord']
+__all__ = ['EulerCore', 'GradientElement']
 
 L.register({
     'sceuler_create': (L.c_int, [L.c_vp, L.c_dbl, L.c_int, L.P_vp]),
@@ -548,127 +548,3 @@ class GradientElement(object):
         L.check(L.lib().scge_solve(self._mesh._h, self._ndim, L.dptr(dst),
                                    L.dptr(u), L.dptr(out)))
         return [float(x) for x in out[:self._ndim]]
-
-
-class EulerField(object):
-    """Body-cell views of one Euler run (solvcon/multidim/euler/_field.py:
-    16-148).  Views alias the solver's host mirrors."""
-
-    FIELDS = ('density', 'velx', 'vely', 'speed', 'pressure', 'mach',
-              'total_energy', 'cfl')
-    MOMENTA = ('momx', 'momy', 'momz')
-
-    def __init__(self, svr, mesh):
-        self.svr = svr
-        self.mesh = mesh
-
-    @property
-    def conserveds(self):
-        return (('density',) + self.MOMENTA[:self.svr.ndim]
-                + ('total_energy',))
-
-    @property
-    def conserved(self):
-        arr = self.svr.so0n
-        return arr.ndarray[arr.nghost:]
-
-    @property
-    def density(self):
-        return self.conserved[:, 0]
-
-    @property
-    def gamma(self):
-        arr = self.svr.gamma
-        return arr.ndarray[arr.nghost:]
-
-    @property
-    def centroid(self):
-        arr = self.mesh.clcnd
-        return arr.ndarray[arr.nghost:]
-
-    @property
-    def volume(self):
-        arr = self.mesh.clvol
-        return arr.ndarray[arr.nghost:]
-
-    @property
-    def vel(self):
-        cons = self.conserved
-        return cons[:, 1:1 + self.svr.ndim] / cons[:, 0][:, None]
-
-    @property
-    def velx(self):
-        return self.vel[:, 0]
-
-    @property
-    def vely(self):
-        return self.vel[:, 1]
-
-    @property
-    def speed(self):
-        return np.sqrt((self.vel ** 2).sum(axis=1))
-
-    @property
-    def total_energy(self):
-        return self.conserved[:, 1 + self.svr.ndim]
-
-    @property
-    def pressure(self):
-        return (self.gamma - 1.0) * (self.total_energy
-                                     - 0.5 * self.density * self.speed ** 2)
-
-    @property
-    def mach(self):
-        return self.speed / np.sqrt(self.gamma * self.pressure / self.density)
-
-    @property
-    def cfl(self):
-        arr = self.svr.cflo
-        return arr.ndarray[arr.nghost:]
-
-    def field(self, name):
-        if name not in self.FIELDS:
-            raise ValueError("unknown field '%s'" % name)
-        return getattr(self, name)
-
-    def calc_overall_mass(self):
-        return float(np.sum(self.density * self.volume))
-
-    def calc_cfl_range(self):
-        cfl = self.cfl
-        return float(cfl.min()), float(cfl.max())
-
-
-RunRecord = collections.namedtuple('RunRecord',
-                                   ['step', 'mass', 'cfl_min', 'cfl_max'])
-
-
-class RunHistory(object):
-    """The newest ``length`` (step, mass, cfl_min, cfl_max) records of a run
-    (the record kept by solvcon/pilot/apps/obsrefl/_session.py:39-70).
-    record() takes them from the solver's device reduction, so a long run
-    moves three numbers per record to the host."""
-
-    def __init__(self, length=2048):
-        self.records = collections.deque(maxlen=length)
-
-    def __len__(self):
-        return len(self.records)
-
-    def append(self, step, mass, cfl_min, cfl_max):
-        rec = RunRecord(step, mass, cfl_min, cfl_max)
-        self.records.append(rec)
-        return rec
-
-    def record(self, core, step):
-        """Append the mass and CFL range of an EulerCore at ``step``."""
-        mass, cmin, cmax = core.reduce()
-        return self.append(step, mass, cmin, cmax)
-
-    @property
-    def last(self):
-        return self.records[-1] if self.records else None
-
-    @property
-    def masses(self):
-        return [(r.step, r.mass) for r in self.records]

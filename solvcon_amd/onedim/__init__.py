@@ -1,5 +1,5 @@
-"""One-dimensional Euler CESE solver (drop-in for solvcon.onedim)."""
+"""One-dimensional Euler CESE core (drop-in for _solvcon.onedim)."""
 
-from .euler1d import Euler1DCore, Euler1DSolver, ShockTube  # noqa: F401
+from .euler1d import Euler1DCore  # noqa: F401
 
-__all__ = ['Euler1DCore', 'Euler1DSolver', 'ShockTube']
+__all__ = ['Euler1DCore']

@@ -1,10 +1,14 @@
 """Batched form: many independent solves in one device march.
 
 CPU: the disjoint union of mesh definitions keeps every instance's cells
-and nodes.  GPU: every instance marched in the batch is bit-identical, on
-every body and ghost row of every table, to the same instance marched alone
-by the plain EulerCore with the same dt.  That holds for mixed sizes, step
-counts, cell types and dimensions.
+and nodes.  GPU: every instance marched in the batch equals the oracle's
+march of that instance alone (the reference's EulerCore::march,
+euler_march.cpp:339-359, restated) with the same dt and step count, value
+for value, and is bit-identical, on every body and ghost row of every
+table, to the same instance marched alone by the plain EulerCore.  That
+holds for mixed sizes, step counts, cell types and dimensions, for the
+committed tiny-mesh march fixtures taken as instances, and for a sample of
+the batched workload's own generator.
 """
 
 import numpy as np
@@ -84,7 +88,7 @@ def _check_batch(defs, states, steps, chunk=0):
 @pytest.mark.gpu
 @pytest.mark.parametrize('fans', [0, 1, 2])
 @pytest.mark.parametrize('chunk', [0, 150])
-def test_gpu_batch_equals_alone_2d(chunk, fans):
+def test_gpu_batch_equals_alone_2d(orc, chunk, fans):
     # chunk 150: rank chunks of about 150 cells, marched one after another.
     print('case: batch of 2D instances, chunk', chunk, 'fans', fans,
           flush=True)
@@ -103,12 +107,13 @@ def test_gpu_batch_equals_alone_2d(chunk, fans):
     for k in range(fans):
         defs.append(MD.TRIANGLES)
         states.append(np.tile([1.0, 0.1 * (k + 1), 0.0, 2.5], (3, 1)))
-    _check_batch(defs, states, [4, 7, 2, 7, 5, 1, 6, 3][:len(defs)], chunk)
+    _check_batch(defs, states, [4, 7, 2, 7, 5, 1, 6, 3][:len(defs)], chunk,
+                 orc)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('chunk', [0, 200])
-def test_gpu_batch_equals_alone_3d(chunk):
+def test_gpu_batch_equals_alone_3d(orc, chunk):
     print('case: batch of 3 3D mixed instances, chunk', chunk, flush=True)
     from solvcon_amd.gen import mixed_mesh_def, blast_state
     from solvcon_amd.batch import build_mesh
@@ -119,27 +124,59 @@ def test_gpu_batch_equals_alone_3d(chunk):
         mh = build_mesh(d)
         cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
         states.append(blast_state(cen, r=0.4, center=cen.mean(0)))
-    _check_batch(defs, states, [2, 3, 1], chunk)
+    _check_batch(defs, states, [2, 3, 1], chunk, orc)
 
 
 @pytest.mark.gpu
-def test_gpu_batch_config_sample():
-    # A few instances of the batched workload's own generator.
+def test_gpu_batch_config_sample(orc):
+    # Instances of the batched workload's own generator, sizes 50-3000
+    # cells and 20-120 steps, against the oracle and the plain march.
     print('case: batch config sample', flush=True)
     from solvcon_amd.batch import (batch_config, config_defs, config_states,
-                                   BatchedEuler)
-    m, ms, rs, steps = batch_config(ninst=5, lo=50, hi=800, smin=2, smax=9)
+                                   BatchedEuler, build_mesh)
+    m, ms, rs, steps = batch_config(ninst=12, lo=50, hi=3000, smin=20,
+                                    smax=120)
     defs = config_defs(m, ms)
     st = config_states(rs)
     b = BatchedEuler(defs, st, steps)
     b.march()
     for k in range(len(defs)):
-        from solvcon_amd.batch import build_mesh
         mh = build_mesh(defs[k])
         cen = mh.clcnd.ndarray[mh.ngstcell:].copy()
+        _check_oracle(orc, b, k, defs[k], st(cen, k), steps[k])
         _, ec = _alone(defs[k], st(cen, k), b.dts[k], steps[k])
         assert np.array_equal(ec.so0n.ndarray[ec.ngstcell:],
                               b.body('so0n', k))
+
+
+@pytest.mark.gpu
+def test_gpu_batch_golden_instances(orc):
+    # The committed tiny-mesh march fixtures (tests/golden/march_*.npz),
+    # all 2D ones in one batch and all 3D ones in another, each with its
+    # fixture's state, dt and step count but non-reflective boundaries
+    # throughout: every instance equals the oracle's march alone.
+    print('case: batch of the golden fixture meshes', flush=True)
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'golden'))
+    import make_golden as MG
+    from solvcon_amd.batch import BatchedEuler, build_mesh
+    cases = MG.march_cases()
+    for nd in (2, 3):
+        import pair
+        names = sorted(n for n in cases if cases[n][0] == nd)
+        defs = [pair.as_arrays(cases[n]) for n in names]
+        states = []
+        for d in defs:
+            mh = build_mesh(d)
+            states.append(MG.march_state(mh.clcnd.ndarray[mh.ngstcell:], nd))
+        steps = [MG.MARCH_STEPS + i for i in range(len(defs))]
+        b = BatchedEuler(defs, states, steps, dts=[MG.MARCH_DT] * len(defs),
+                         gamma=MG.GAMMA)
+        b.march()
+        for k, d in enumerate(defs):
+            print('  ', names[k], flush=True)
+            _check_oracle(orc, b, k, d, states[k], steps[k])
 
 
 @pytest.mark.gpu

@@ -1,12 +1,17 @@
-"""Parity at the benchmark configurations' own meshes.
+"""Full-run parity at the benchmark configurations' own meshes.
 
-The 2D configuration (1024 x 1024 quads split into 2,097,152 jittered
-triangles, Riemann box, non-reflective boundaries, dt from the CFL 0.8 rule)
-is marched ten steps by the device and by the oracle on all host cores; every
-row of the solution tables must be equal.  The 3D configuration's generator
-and state (blast, slip walls on three sides) are checked the same way at 60^3
-(525,600 cells, five steps): its full 19.3M-cell mesh takes minutes to build
-on the CPU.
+- The 2D configuration (1024 x 1024 quads split into 2,097,152 jittered
+  triangles, Riemann box, non-reflective boundaries, dt from the CFL 0.8
+  rule) is marched its whole 500 steps by the device and by the oracle on
+  all host cores.
+- The 3D configuration's generator and state (mixed hex/tet/prism/pyramid
+  block, blast, slip walls on three sides) are marched the configuration's
+  whole 200 steps at 60^3 (525,600 cells): the full 19.3M-cell mesh takes
+  minutes to build on the CPU, and its oracle march hours.
+
+The bar is a relative error of at most 1e-12 in every column of so0n,
+so1n, cflo and cflc (max |device - oracle| / max |oracle| per column); the
+tests also report whether every value is equal, which they are.
 """
 
 import numpy as np
@@ -48,21 +53,31 @@ def _run(orc, defn, state, bcs, steps):
     oe.call('bc_dsoln')
     print('case: oracle march', flush=True)
     oe.march(steps)
+    worst = 0.0
     for t in ('so0n', 'so1n', 'cflo', 'cflc'):
         a = getattr(ec, t).ndarray
         b = getattr(oe, t)
         assert a.shape == b.shape, t
-        if not np.array_equal(a, b):
-            bad = np.argwhere(a != b)
-            raise AssertionError('%s differs at %d places, first %s'
-                                 % (t, len(bad), tuple(bad[0])))
+        assert np.isfinite(b).all(), t
+        a2 = a.reshape(a.shape[0], -1)
+        b2 = b.reshape(b.shape[0], -1)
+        scale = np.maximum(np.abs(b2).max(axis=0), 1e-300)
+        rel = float((np.abs(a2 - b2).max(axis=0) / scale).max())
+        nbad = int((a != b).sum())
+        print('case: %s max relative error %.3g, %d values differ'
+              % (t, rel, nbad), flush=True)
+        assert rel <= 1e-12, t
+        worst = max(worst, rel)
+    return worst
 
 
-def test_2d_config_mesh_bitwise(orc):
+@pytest.mark.timeout(900)
+def test_2d_config_full_run(orc):
     defn, state, bcs, _ = bench.workload_2d(1024)
-    _run(orc, defn, state, bcs, 10)
+    assert _run(orc, defn, state, bcs, 500) == 0.0
 
 
-def test_3d_config_generator_bitwise(orc):
+@pytest.mark.timeout(900)
+def test_3d_config_generator_full_run(orc):
     defn, state, bcs, _ = bench.workload_3d(60)
-    _run(orc, defn, state, bcs, 5)
+    assert _run(orc, defn, state, bcs, 200) == 0.0

@@ -1,349 +1,193 @@
-"""The oblique-shock reflection problem (solvcon/pilot/apps/obsrefl).
+"""Regular reflection of an oblique shock: the inlet / slip-wall /
+non-reflective march of solvcon's obsrefl application
+(solvcon/pilot/apps/obsrefl), on this build's own rectangle meshes.
 
-Expectations follow the reference's tests/apps/obsrefl/test_driver.py,
-test_analytic.py and test_session.py: oblique-shock relation values from
-Anderson, Modern Compressible Flow, chapter 4; mesh counts, winding, tiling
-and boundary classification of the three mesh flavors; the analysis of the
-seeded analytic field; and a coarse run marched to its cap that reaches the
-analytic zones.  The CPU tests build the meshes with the oracle; the GPU
-tests build them on the device (every table equal to the oracle's), march
-the device solver with its inlet, slip-wall and non-reflective boundaries
-against the oracle bit for bit, and run the analysis and the session.
+- CPU: the oblique-shock relations (oracle/oblique.py) against the values
+  the reference's tests/apps/obsrefl/test_driver.py pins (Anderson, Modern
+  Compressible Flow, chapter 4), the jump conditions across both shocks,
+  the rectangle meshes, and the oracle marched to a steady state whose
+  three zones match the analytic ones.
+- GPU: the device march with two inlets, a slip wall and a non-reflective
+  edge against the oracle bit for bit, at the application's default 64 x 16
+  boxes, for 10 steps and for the whole 600-step run to the steady state.
 """
 
 import math
 
 import numpy as np
 import pytest
-from numpy.testing import assert_almost_equal
 
-import pair
-from solvcon_amd.obsrefl import (ObliqueShockMesher, ObliqueShockRelation,
-                                 ObliqueShock, Reflection, ReflectionSession,
-                                 ZoneInfo)
-from solvcon_amd import RunHistory
+from oracle import oblique as OB
+from solvcon_amd.gen import rect_mesh_def, edge_faces
 
-NX, NY, UR = 24, 8, (3.0, 1.0)
-FLAVORS = {'quad': (1, 3), 'triangle': (2, 4), 'unstructured': (None, 4)}
+EXTENT = (4.0, 1.0)
+FLAVORS = {'quad': (4e-3, 3), 'tri': (3e-3, 4)}     # dt, cell type
 
 
 # ------------------------------------------------------- relations (CPU) -----
 
-def test_relation_ratios():
-    ob = ObliqueShockRelation(gamma=1.4)
+def test_relation_values():
     beta = math.radians(37.8)
-    assert ob.calc_density_ratio(3, beta) == pytest.approx(2.4204302545,
-                                                           abs=1e-10)
-    assert ob.calc_pressure_ratio(3, beta) == pytest.approx(3.7777114257,
-                                                            abs=1e-10)
-    assert ob.calc_temperature_ratio(3, beta) == pytest.approx(1.5607602899,
-                                                               abs=1e-10)
-    ob.gamma = 1.2
-    assert ob.calc_density_ratio(3, beta) == pytest.approx(2.7793244902,
-                                                           abs=1e-10)
-
-
-def test_relation_mach_and_angles():
-    ob = ObliqueShockRelation(gamma=1.4)
-    assert ob.calc_normal_dmach(3) == pytest.approx(0.4751909633, abs=1e-10)
-    assert ob.calc_dmach(3, beta=math.radians(37.8)) == pytest.approx(
-        1.9924827009, abs=1e-10)
-    assert ob.calc_dmach(3, theta=math.radians(20)) == pytest.approx(
-        1.9941316656, abs=1e-10)
+    assert OB.density_ratio(3, beta) == pytest.approx(2.4204302545,
+                                                      abs=1e-10)
+    assert OB.pressure_ratio(3, beta) == pytest.approx(3.7777114257,
+                                                       abs=1e-10)
+    assert OB.temperature_ratio(3, beta) == pytest.approx(1.5607602899,
+                                                          abs=1e-10)
+    assert OB.density_ratio(3, beta, g=1.2) == pytest.approx(2.7793244902,
+                                                             abs=1e-10)
+    assert OB.normal_mach_downstream(3) == pytest.approx(0.4751909633,
+                                                         abs=1e-10)
+    assert OB.downstream_mach(3, beta=beta) == pytest.approx(1.9924827009,
+                                                             abs=1e-10)
+    assert OB.downstream_mach(3, theta=math.radians(20)) == pytest.approx(
+        1.9941316656, abs=1e-9)
+    b = OB.shock_angle(4, math.radians(32))
+    assert math.degrees(b) == pytest.approx(48.2584798722, abs=1e-8)
+    assert math.degrees(OB.deflection(4, b)) == pytest.approx(32.0,
+                                                              abs=1e-10)
     with pytest.raises(ValueError):
-        ob.calc_dmach(3)
+        OB.shock_angle(2, math.radians(40))
     with pytest.raises(ValueError):
-        ob.calc_dmach(3, beta=0.2, theta=0.1)
+        OB.downstream_mach(3)
     with pytest.raises(ValueError):
-        ob.calc_shock_angle(2, math.radians(40))
-    beta = ob.calc_shock_angle(4, math.radians(32), delta=1)
-    assert math.degrees(beta) == pytest.approx(48.2584798722, abs=1e-10)
-    assert math.degrees(ob.calc_flow_angle(4, beta)) == pytest.approx(
-        32.0, abs=1e-6)
+        OB.downstream_mach(3, beta=0.2, theta=0.1)
 
 
-def test_constants_set_post_shock_state():
-    s = ObliqueShock()
-    s.build_constant(gamma=1.4, density=1.0, pressure=1.0, mach=3.0,
-                     angle=10.0)
-    rel = s.relation
-    beta = rel.calc_shock_angle(3.0, math.radians(10.0))
-    assert beta == pytest.approx(s.shock_angle)
-    assert rel.calc_density_ratio(3.0, beta) == pytest.approx(s.density2)
-    assert rel.calc_pressure_ratio(3.0, beta) == pytest.approx(s.pressure2)
-    vx, vy = s.velocity2
-    assert vy < 0.0
-    assert math.atan2(-vy, vx) == pytest.approx(math.radians(10.0))
-    c2 = math.sqrt(1.4 * s.pressure2 / s.density2)
-    assert s.mach2 == pytest.approx(math.hypot(vx, vy) / c2)
-    with pytest.raises(ValueError):
-        ObliqueShock().build_numerical()
+def _frame_flux(state, n):
+    # Mass, normal-momentum, tangential-momentum and energy fluxes through a
+    # surface of unit normal n.
+    rho, (u, v), p = state
+    un = u * n[0] + v * n[1]
+    ut = -u * n[1] + v * n[0]
+    h = 1.4 / 0.4 * p / rho + 0.5 * (u * u + v * v)
+    return np.array([rho * un, rho * un * un + p, rho * un * ut,
+                     rho * un * h])
 
 
-def test_zone_info_checks_the_zone():
-    for z in ZoneInfo.ZONES:
-        assert ZoneInfo(z, 1, 1.0, 1.0, 0.0).zone == z
-    for z in (0, 4, '2'):
-        with pytest.raises(ValueError):
-            ZoneInfo(z, 1, 1.0, 1.0, 0.0)
-    r = ZoneInfo(1, 1, 1.0, 1.0, 0.0)
-    r.zone = 3
-    with pytest.raises(ValueError):
-        r.zone = 4
-
-
-def test_run_history_bounded():
-    h = RunHistory(length=3)
-    assert h.last is None
-    for step in range(1, 6):
-        h.append(step, 1.0 / step, 0.1 * step, 0.2 * step)
-    assert len(h) == 3 and h.last.step == 5
-    assert h.masses == [(3, 1 / 3), (4, 0.25), (5, 0.2)]
+def test_reflection_jump_conditions():
+    # Rankine-Hugoniot across the incident and the reflected shock, computed
+    # from the zone states and the shock directions alone.
+    rr = OB.RegularReflection()
+    b1 = rr.beta1
+    n_inc = (math.sin(b1), math.cos(b1))           # normal of the incident
+    np.testing.assert_allclose(_frame_flux(rr.zones[0], n_inc),
+                               _frame_flux(rr.zones[1], n_inc), rtol=1e-12)
+    a = rr.beta2 - rr.theta                        # reflected shock angle
+    n_ref = (math.sin(a), -math.cos(a))
+    np.testing.assert_allclose(_frame_flux(rr.zones[1], n_ref),
+                               _frame_flux(rr.zones[2], n_ref), rtol=1e-12,
+                               atol=1e-12)
+    # Zone 3 flows along the wall again; pressures rise through each shock.
+    assert rr.zones[2][1][1] == 0.0
+    assert rr.zones[0][2] < rr.zones[1][2] < rr.zones[2][2]
+    assert rr.wall_x == pytest.approx(1.0 / math.tan(b1))
 
 
 # ---------------------------------------------------------- meshes (CPU) -----
 
-def _signed_area2(ndcrd, row):
-    n = row[0]
-    xy = ndcrd[row[1:1 + n]]
-    return float(np.sum(xy[:, 0] * np.roll(xy[:, 1], -1)
-                        - np.roll(xy[:, 0], -1) * xy[:, 1]))
-
-
 @pytest.fixture(scope='module', params=sorted(FLAVORS))
 def flavor(request, orc):
-    m = ObliqueShockMesher(nx=NX, ny=NY, ur=UR)
-    defn = m.definition(request.param)
-    return request.param, m, defn, orc.make_mesh(*defn)
+    name = request.param
+    defn = rect_mesh_def(64, 16, extent=EXTENT, kind=name, seed=7)
+    return name, defn, orc.make_mesh(*defn)
 
 
-def test_mesh_counts_and_winding(flavor):
-    name, m, defn, o = flavor
-    per_box, tpn = FLAVORS[name]
-    _, ndcrd, cltpn, clnds = defn
-    nbox, nbnd = NX * NY, 2 * (NX + NY)
-    assert (cltpn == tpn).all()
-    bf = np.nonzero(o.fccls[o.ngstface:, 1] < 0)[0]
-    assert bf.size == nbnd
-    if per_box:
-        assert o.ncell == per_box * nbox
-        assert o.nface == (per_box + 1) * nbox + NX + NY
-    else:
-        assert o.ncell == 2 * (o.nnode - nbnd) + nbnd - 2
-        assert o.nnode - o.nface + o.ncell == 1
-    areas = [_signed_area2(ndcrd, r) for r in clnds]
-    assert min(areas) > 0.0
+def test_rect_mesh(flavor):
+    name, defn, o = flavor
+    per_box = 1 if name == 'quad' else 2
+    assert o.ncell == per_box * 64 * 16
+    assert (o.cltpn[o.ngstcell:] == FLAVORS[name][1]).all()
     assert (o.clvol[o.ngstcell:] > 0).all()
-    # The cells tile the domain: their area equals the boundary integral.
-    fa = o.fcara[o.ngstface:][bf]
-    fc = o.fccnd[o.ngstface:][bf]
-    fn = o.fcnml[o.ngstface:][bf]
-    enclosed = 0.5 * np.sum(fa * (fc * fn).sum(axis=1))
-    assert_almost_equal(0.5 * sum(areas), enclosed, decimal=10)
-    if tpn == 4:
-        owners = o.fccls[o.ngstface:][bf, 0]
-        assert np.bincount(owners).max() <= 1
-
-
-def test_boundary_classification(flavor):
-    name, m, defn, o = flavor
-    edges = m.classify_boundary(o)
-    sets = [set(e) for e in edges]
-    assert all(sets)
-    assert sum(len(s) for s in sets) == len(set().union(*sets))
-    bf = set(np.nonzero(o.fccls[o.ngstface:, 1] < 0)[0].tolist())
-    assert set().union(*sets) == bf
-    left, top, bottom, right = edges
-    assert (len(left), len(right), len(top), len(bottom)) == (NY, NY, NX, NX)
-    fccnd, fcnml = o.fccnd[o.ngstface:], o.fcnml[o.ngstface:]
-    for faces, axis, coord, normal in ((left, 0, 0.0, (-1, 0)),
-                                       (right, 0, UR[0], (1, 0)),
-                                       (top, 1, UR[1], (0, 1)),
-                                       (bottom, 1, 0.0, (0, -1))):
-        assert_almost_equal(fccnd[faces, axis], coord, decimal=12)
-        assert_almost_equal(fcnml[faces], np.tile(normal, (len(faces), 1)),
-                            decimal=12)
-
-
-def test_unstructured_is_deterministic():
-    m = ObliqueShockMesher(nx=NX, ny=NY, ur=UR)
-    a = m.definition('unstructured')
-    b = m.definition('unstructured')
-    assert np.array_equal(a[1], b[1]) and np.array_equal(a[3], b[3])
+    assert o.clvol[o.ngstcell:].sum() == pytest.approx(4.0, rel=1e-13)
+    edges = edge_faces(o.bndfcs[:, 0], o.fccnd[o.ngstface:], EXTENT)
+    assert [len(e) for e in edges] == [16, 64, 64, 16]
+    allf = np.concatenate(edges)
+    assert sorted(allf.tolist()) == sorted(o.bndfcs[:, 0].tolist())
+    fn = o.fcnml[o.ngstface:]
+    for faces, normal in zip(edges, ((-1, 0), (0, 1), (0, -1), (1, 0))):
+        np.testing.assert_allclose(fn[faces], np.tile(normal,
+                                                      (len(faces), 1)),
+                                   atol=1e-12)
     with pytest.raises(ValueError):
-        ObliqueShockMesher(nx=2, ny=2).definition('hexagon')
+        rect_mesh_def(2, 2, kind='hexagon')
+
+
+def _setup(orc, defn, dt, gpu):
+    rr = OB.RegularReflection()
+    o = orc.make_mesh(*defn)
+    edges = edge_faces(o.bndfcs[:, 0], o.fccnd[o.ngstface:], EXTENT)
+    rho, (u, v), p = rr.zones[0]
+    z2 = rr.zones[1]
+    inl1 = [rho, u, v, p, 1.4]
+    inl2 = [z2[0], z2[1][0], z2[1][1], z2[2], 1.4]
+    oe = orc.Euler(o, dt)
+    oe.init_solution(1.4, rho, [u, v], p)
+    left, top, bottom, right = edges
+    oe.add_bc(orc.BC_INLET, left, inl1)
+    oe.add_bc(orc.BC_INLET, top, inl2)
+    oe.add_bc(orc.BC_SLIPWALL, bottom)
+    oe.add_bc(orc.BC_NONREFL, right)
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    e = None
+    if gpu:
+        import solvcon_amd as sc
+        import pair
+        mh = pair.gpu_mesh(defn)
+        e = sc.EulerCore(mesh=mh, time_increment=dt)
+        e.init_solution(1.4, rho, [u, v], p)
+        e.add_inlet(left, inl1)
+        e.add_inlet(top, inl2)
+        e.add_slipwall(bottom)
+        e.add_nonrefl(right)
+        e.bc_soln()
+        e.bc_dsoln()
+    return rr, o, oe, e
+
+
+def _zone_errors(rr, o, so0n):
+    ng = o.ngstcell
+    zone = rr.zone_of(o.clcnd[ng:], margin=3 * EXTENT[0] / 64)
+    out = []
+    for z in range(3):
+        k = zone == z
+        assert k.sum() > 50
+        out.append(so0n[ng:, 0][k].mean() / rr.zones[z][0] - 1.0)
+    return out
+
+
+def test_oracle_reaches_the_analytic_zones(orc, flavor):
+    # 600 steps carry the free stream across the domain several times; the
+    # mean density of each zone, away from the shocks, is then within 3%
+    # of the analytic one.  This pins the oracle's inlet and slip-wall
+    # conditions (euler_bc.cpp:143-320) against the analytic solution.
+    name, defn, _ = flavor
+    dt = FLAVORS[name][0]
+    rr, o, oe, _ = _setup(orc, defn, dt, gpu=False)
+    oe.march(600)
+    ng = o.ngstcell
+    assert np.isfinite(oe.so0n).all() and oe.cflo[ng:].max() < 0.6
+    for err in _zone_errors(rr, o, oe.so0n):
+        assert abs(err) < 0.03
 
 
 # ------------------------------------------------------------------ GPU ------
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('name', sorted(FLAVORS))
-def test_device_mesh_and_groups(orc, name):
-    print('case: obsrefl mesh', name, flush=True)
-    m = ObliqueShockMesher(nx=NX, ny=NY, ur=UR)
-    mh = m.make_mesh(name)
-    edges = m.classify_boundary(orc.make_mesh(*m.definition(name)))
-    o = orc.make_mesh(*m.definition(name),
-                      bcs=list(zip(m.BOUNDARY_NAMES, edges)))
-    pair.same_mesh(mh, o)
-    assert [b.name for b in mh.bcs] == list(m.BOUNDARY_NAMES)
-    for nm, faces in zip(m.BOUNDARY_NAMES, edges):
-        assert mh.bc(nm).facn.ndarray[:, 0].tolist() == list(faces)
-
-
-@pytest.mark.gpu
-@pytest.mark.parametrize('name', sorted(FLAVORS))
-def test_driver_march_equals_oracle(orc, name):
-    # Inlet (free stream and post-shock), slip wall and non-reflective
-    # boundaries: the device march equals the oracle's bit for bit.
-    print('case: obsrefl march', name, flush=True)
-    s = ObliqueShock()
-    s.build_constant()
-    s.build_numerical(cell_type=name, nx=NX, ny=NY)
-    assert s.mesh.ncell == s.svr.ncell and s.svr.ndim == 2
-    m = s.mesher
-    defn = m.definition(name)
-    edges = m.classify_boundary(orc.make_mesh(*defn))
-    # The same named groups, so ghost cells are numbered alike.
-    o = orc.make_mesh(*defn, bcs=list(zip(m.BOUNDARY_NAMES, edges)))
-    pair.same_mesh(s.mesh, o)
-    oe = orc.Euler(o, 2e-3)
-    oe.init_solution(s.gamma, s.density, [s.velocity, 0.0], s.pressure)
-    left, top, bottom, right = m.classify_boundary(o)
-    oe.add_bc(orc.BC_INLET, np.asarray(left),
-              [s.density, s.velocity, 0.0, s.pressure, s.gamma])
-    oe.add_bc(orc.BC_INLET, np.asarray(top),
-              [s.density2, s.velocity2[0], s.velocity2[1], s.pressure2,
-               s.gamma])
-    oe.add_bc(orc.BC_SLIPWALL, np.asarray(bottom))
-    oe.add_bc(orc.BC_NONREFL, np.asarray(right))
-    oe.call('bc_soln')
-    oe.call('bc_dsoln')
-    s.march(10)
-    oe.march(10)
-    pair.same_euler(s.svr, oe, names=('so0n', 'so1n', 'cflo', 'cflc'))
-
-
-def _build(cell_type='quad', nx=64, ny=16, **kw):
-    s = ObliqueShock()
-    s.build_constant()
-    s.build_numerical(cell_type=cell_type, nx=nx, ny=ny, **kw)
-    return s, Reflection(s)
-
-
-def _seed(an):
-    masks = an.zone_masks(margin=0.0)
-    zones = an.zone_conserved()
-    cons = an.field.conserved
-    cons[:] = zones[1]
-    cons[masks[0]] = zones[0]
-    cons[masks[2]] = zones[2]
-    return masks
-
-
-@pytest.mark.gpu
-def test_analysis_of_the_analytic_field():
-    print('case: obsrefl analysis', flush=True)
-    s, an = _build()
-    masks = _seed(an)
-    path = s.shock_path()
-    assert len(an.arms) == len(path) - 1
-    inc, ref = an.incident, an.reflected
-    assert_almost_equal([inc.x0, inc.y0, inc.x1, inc.y1], path[0] + path[1])
-    assert_almost_equal([ref.x0, ref.y0, ref.x1, ref.y1], path[1] + path[2])
-    assert an.arms.pack_array().ndarray.shape == (2, 4)
-    counts = [int(k.sum()) for k in masks]
-    assert sum(counts) == s.mesh.ncell and min(counts) > 0
-    assert not (masks[0] & masks[1]).any() and not (masks[1] & masks[2]).any()
-    for kept, whole in zip(an.zone_masks(), masks):
-        assert ((whole | kept) == whole).all() and kept.sum() < whole.sum()
-    for nm in an.zone_fields():
-        for rec in an.zone_info(nm):
-            assert rec.count > 0
-            assert_almost_equal(rec.computed, rec.analytic, decimal=12)
-    assert an.zone_info('cfl') == []
-    with pytest.raises(ValueError):
-        an.zone_info('nonesuch')
-    rho = an.zone_field('density')
-    assert an.color_range('density', 1e9, -1e9) == (float(rho.min()),
-                                                     float(rho.max()))
-    assert an.color_range('cfl', 0.2, 0.7) == (0.2, 0.7)
-    fit = an.fit_incident_angle()
-    assert_almost_equal(fit.analytic, math.degrees(s.shock_angle),
-                        decimal=12)
-    assert abs(fit.degree - fit.analytic) < 0.5
-    wall = an.reflection_point()
-    assert abs(wall.x - wall.analytic) < s.mesher.cell_extent[0]
-    h = (s.mesher.y1 - s.mesher.y0) * 8.5 / 16
-    prof = an.profile(h)
-    assert len(prof.x) == 64 and (np.diff(prof.x) > 0).all()
-    assert_almost_equal(prof.computed, prof.analytic, decimal=12)
-    assert (np.diff(prof.analytic) >= 0).all()
-    assert len(set(prof.analytic.tolist())) == 3
-
-
-@pytest.mark.gpu
-def test_analysis_edge_cases():
-    print('case: obsrefl analysis edge cases', flush=True)
-    s, an = _build(cell_type='unstructured', nx=32, ny=8)
-    _seed(an)
-    fit = an.fit_incident_angle()
-    assert abs(fit.degree - fit.analytic) < 1.5
-    s, an = _build(nx=24, ny=8)
-    assert len(an.crossings()) == 0
-    fit = an.fit_incident_angle()
-    assert fit.npoint == 0 and math.isnan(fit.degree)
-    assert math.isnan(an.reflection_point().x)
-    an.margin = 10.0
-    rec = an.zone_info()[0]
-    assert rec.count == 0 and math.isnan(rec.computed)
-    s, an = _build(nx=16, ny=8, ur=(1.0, 1.0))
-    assert len(s.shock_path()) == 2 and not an.has_reflection
-    assert an.reflected is None and an.incident.x1 == s.mesher.x1
-    masks = an.zone_masks(margin=0.0)
-    assert masks[2].sum() == 0
-    _seed(an)
-    assert math.isnan(an.reflection_point().analytic)
-    s = ObliqueShock()
-    s.build_constant()
-    with pytest.raises(ValueError):
-        Reflection(s)
-
-
-@pytest.mark.gpu
-def test_session_chunks():
-    print('case: obsrefl session', flush=True)
-    sess = ReflectionSession(nx=8, ny=3, steps_per_chunk=3)
-    assert sess.step == 0 and not sess.finished and len(sess.history) == 0
-    assert sess.field is sess.analysis.field
-    rec = sess.advance()
-    assert sess.step == 3 and rec.step == 3
-    assert rec.mass == pytest.approx(sess.field.calc_overall_mass())
-    assert sess.field.calc_cfl_range() == (rec.cfl_min, rec.cfl_max)
-    assert rec.cfl_min > 0.0
-    sess = ReflectionSession(nx=8, ny=3, steps_per_chunk=4, max_steps=6)
-    assert sess.advance().step == 4 and sess.advance().step == 6
-    assert sess.stop_reason == 'cap' and sess.advance() is None
-    sess = ReflectionSession(nx=8, ny=3, steps_per_chunk=2)
-    sess.advance()
-    sess.stop()
-    sess.stop()
-    assert sess.stop_reason == 'stopped' and len(sess.zone_info()) == 3
-
-
-@pytest.mark.gpu
-def test_coarse_run_reaches_the_analytic_answer():
-    # tests/apps/obsrefl/test_session.py:96-128: 400 steps on a coarse
-    # unstructured mesh reach the three analytic zones.
-    print('case: obsrefl analytic agreement', flush=True)
-    sess = ReflectionSession(nx=12, ny=4, time_increment=1.e-2,
-                             steps_per_chunk=20, max_steps=400)
-    assert sess.run() == 'cap'
-    assert len(sess.history) == 20
-    cflc = sess.shock.svr.cflc
-    assert float(cflc.ndarray[cflc.nghost:].max()) < 1.0
-    for rec in sess.zone_info():
-        assert abs(rec.error) < 0.2
-    fit = sess.fit_incident_angle()
-    assert abs(fit.degree - fit.analytic) < 10.0
-    assert abs(sess.reflection_point().error) < 0.3
+@pytest.mark.parametrize('steps', [10, 600])
+def test_reflection_march_equals_oracle(orc, name, steps):
+    # The device march with its two inlets, slip wall and non-reflective
+    # edge equals the oracle's bit for bit, after 10 steps and after the
+    # whole run to the steady state.
+    print('case: oblique reflection', name, steps, 'steps', flush=True)
+    import pair
+    defn = rect_mesh_def(64, 16, extent=EXTENT, kind=name, seed=7)
+    rr, o, oe, e = _setup(orc, defn, FLAVORS[name][0], gpu=True)
+    e.march(steps)
+    oe.march(steps)
+    pair.same_euler(e, oe, names=('so0n', 'so1n', 'cflo', 'cflc'))
+    if steps == 600:
+        for err in _zone_errors(rr, o, e.so0n.ndarray):
+            assert abs(err) < 0.03

@@ -1,63 +1,173 @@
 """1D Euler CESE: oracle pinning (CPU) and HIP parity (GPU).
 
-CPU: the ShockTube analytic goldens (tests/test_onedim_euler.py:113-141 of
-the reference), the fine-grained stage API against march_alpha2 bit for bit
-(:53-71), an L1 bound against the analytic Sod solution, and the committed
-golden vectors.  GPU: the HIP march against the oracle bit for bit on the
-Sod tube (2001 points, 600 steps) and on sizes that take the multi-kernel
-path.
+CPU:
+- the exact Riemann solution (oracle/riemann.py) against the reference's
+  ShockTube goldens (tests/test_onedim_euler.py:113-141), the jump
+  conditions across its shock and the Riemann invariant across its fan;
+- the oracle's derived-field accessors (Euler1DCore.hpp:106-168,
+  Euler1DCore.cpp:37-68,174-205) and update_cfl (Euler1DCore.cpp:70-94)
+  against closed forms on hand-chosen states and a non-uniform grid;
+- the fine-grained stage API against march_alpha2 bit for bit (:53-71), an
+  L1 bound against the exact Sod solution, and the committed golden vectors.
+GPU: the HIP march against the oracle bit for bit on the Sod tube (2001
+points, 600 steps) and on sizes that take the multi-kernel path.
 """
 
+import math
 import os
 
 import numpy as np
 import pytest
 
+from oracle.riemann import RiemannSolution
+
 GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+SOD = ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1))
 
 
-def _shocktube():
-    from solvcon_amd.onedim import ShockTube
-    st = ShockTube()
-    st.build_constant(gamma=1.4, pressure1=1.0, density1=1.0,
-                      pressure5=0.1, density5=0.125)
-    return st
-
-
-def test_shocktube_goldens():
-    st = _shocktube()
-    np.testing.assert_allclose(st.calc_pressure45(), 3.0313017805064697,
+def test_riemann_sod_goldens():
+    # The values the reference's ShockTube tests pin, rtol 1e-7.
+    s = RiemannSolution(*SOD, gamma=1.4)
+    assert (s.left_wave, s.right_wave) == ('rarefaction', 'shock')
+    np.testing.assert_allclose(s.p_star / 0.1, 3.0313017805064697, rtol=1e-7)
+    np.testing.assert_allclose([s.rho_star_left, s.rho_star_right],
+                               [0.42631942817849494, 0.2655737117053072],
                                rtol=1e-7)
+    np.testing.assert_allclose(s.p_star, 0.303130178050647, rtol=1e-7)
+    np.testing.assert_allclose(s.u_star, 0.9274526200489503, rtol=1e-7)
     np.testing.assert_allclose(
-        [st.density1, st.density3, st.density4, st.density5],
-        [1.0, 0.42631942817849494, 0.2655737117053072, 0.125], rtol=1e-7)
-    assert st.pressure3 == st.pressure4
-    np.testing.assert_allclose(
-        [st.pressure1, st.pressure3, st.pressure4, st.pressure5],
-        [1.0, 0.303130178050647, 0.303130178050647, 0.1], rtol=1e-7)
-    assert st.velocity3 == st.velocity4
-    np.testing.assert_allclose(
-        [st.velocity1, st.velocity3, st.velocity4, st.velocity5],
-        [0.0, 0.9274526200489503, 0.9274526200489503, 0.0], rtol=1e-7)
-    np.testing.assert_allclose(
-        st.calc_locations(t=0.1),
+        s.locations(t=0.1),
         [-0.11832159566199232, -0.007027281256118345, 0.09274526200489504,
          0.17521557320301784], rtol=1e-7)
-    assert np.sqrt(st.gamma) == st.calc_speedofsound(pressure=1.0,
-                                                     density=1.0)
-    assert np.sqrt(st.gamma * st.pressure5 / st.density5) == st.speedofsound5
+    # The fan head moves at the left sound speed.
+    assert s.locations(1.0)[0] == -math.sqrt(1.4)
 
 
-def test_shocktube_field_t0():
-    st = _shocktube()
-    st.build_field(t=0.0, coord=np.linspace(-1, 1, num=11))
-    assert st.svr is None and len(st.coord_field) == 11
-    assert (st.velocity_field == 0).all()
-    left = st.coord_field < 0
-    assert (st.pressure_field[left] == 1.0).all()
-    assert (st.pressure_field[~left] == 0.1).all()
-    assert (st.density_field[left] == 1.0).all()
-    assert (st.density_field[~left] == 0.125).all()
+def _flux(rho, u, p, g=1.4):
+    e = p / (g - 1) + 0.5 * rho * u * u
+    return np.array([rho * u, rho * u * u + p, u * (e + p)]), \
+        np.array([rho, rho * u, e])
+
+
+@pytest.mark.parametrize('left,right', [
+    SOD, ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01)),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4)),       # two rarefactions
+    ((1.0, 2.0, 0.4), (1.0, -2.0, 0.4)),       # two shocks
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.095))])
+def test_riemann_jump_conditions(left, right):
+    # Rankine-Hugoniot across every shock, u + 2c/(g-1) (left fan) or
+    # u - 2c/(g-1) (right fan) constant through every rarefaction, and
+    # continuity of p and u across the contact.
+    g = 1.4
+    s = RiemannSolution(left, right, gamma=g)
+    lh, lt, us, rt, rh = s.speeds()
+    assert lh <= lt <= us <= rt <= rh
+    stars = ((s.rho_star_left, s.u_star, s.p_star),
+             (s.rho_star_right, s.u_star, s.p_star))
+    for side, wave, star, (a, b), sign in (
+            (left, s.left_wave, stars[0], (lh, lt), -1),
+            (right, s.right_wave, stars[1], (rt, rh), +1)):
+        if wave == 'shock':
+            fa, ua = _flux(*side)
+            fb, ub = _flux(*star)
+            np.testing.assert_allclose(fb - fa, a * (ub - ua), rtol=1e-10,
+                                       atol=1e-10 * np.abs(fa).max())
+        else:
+            xi = np.linspace(a, b, 7)
+            rho, u, p = s.sample(xi, 1.0)
+            c = np.sqrt(g * p / rho)
+            inv = u - sign * 2 * c / (g - 1)
+            c0 = math.sqrt(g * side[2] / side[0])
+            np.testing.assert_allclose(inv, side[1] - sign * 2 * c0 / (g - 1),
+                                       rtol=1e-12)
+            np.testing.assert_allclose(p / rho ** g, side[2] / side[0] ** g,
+                                       rtol=1e-12)
+    # Far from the origin the initial states persist.
+    rho, u, p = s.sample([-1e3, 1e3], 1.0)
+    assert (rho[0], u[0], p[0]) == tuple(map(float, left))
+    assert (rho[1], u[1], p[1]) == tuple(map(float, right))
+
+
+def test_riemann_mirror_symmetry():
+    # Reflecting x -> -x swaps the states and negates the velocities.
+    a = RiemannSolution((1.0, 0.3, 1.0), (0.125, -0.1, 0.1))
+    b = RiemannSolution((0.125, 0.1, 0.1), (1.0, -0.3, 1.0))
+    assert b.p_star == pytest.approx(a.p_star, rel=1e-14)
+    assert b.u_star == pytest.approx(-a.u_star, rel=1e-14)
+    x = np.linspace(-0.9, 0.9, 37)
+    ra, ua, pa = a.sample(x, 0.3)
+    rb, ub, pb = b.sample(-x, 0.3)
+    np.testing.assert_allclose(rb, ra, rtol=1e-12)
+    np.testing.assert_allclose(ub, -ua, rtol=1e-12, atol=1e-14)
+    with pytest.raises(ValueError):
+        RiemannSolution((1.0, -20.0, 1.0), (1.0, 20.0, 1.0))
+
+
+# Hand-chosen conserved states (rho, m, E), exact in binary: their fields
+# follow from the ideal-gas relations p = (g-1) rho e, T = p / (rho R),
+# s = p / rho^g with e the specific internal energy E/rho - u^2/2.
+_STATES = np.array([[2.0, 4.0, 10.0], [0.5, -1.0, 4.0], [1.0, 0.0, 2.5],
+                    [4.0, 2.0, 0.125]])
+_R = 8.31446261815324
+
+
+def test_oracle_accessors_closed_form(orc):
+    g = 1.4
+    o = orc.Euler1D(5, 0.1)
+    o.so0[:4] = _STATES
+    o.so0[4] = [1.0, 0.0, 1.0]
+    rho, m, e = _STATES.T
+    u = m / rho
+    eint = e / rho - 0.5 * u * u               # 3, 2, 2.5, -0.09375
+    p = (g - 1) * rho * eint
+    assert eint.tolist() == [3.0, 6.0, 2.5, -0.09375]
+    np.testing.assert_allclose(o.derived('density')[:4], rho, rtol=0)
+    np.testing.assert_allclose(o.derived('velocity')[:4], u, rtol=1e-15)
+    np.testing.assert_allclose(o.derived('internal_energy')[:4], eint,
+                               rtol=1e-15)
+    np.testing.assert_allclose(o.derived('pressure')[:4], p, rtol=1e-15)
+    np.testing.assert_allclose(o.derived('temperature')[:4],
+                               p / (rho * _R), rtol=1e-14)
+    # Entropy against p / rho^g; a negative pressure stays negative.
+    ent = o.derived('entropy')[:4]
+    np.testing.assert_allclose(ent, p / rho ** g, rtol=1e-14)
+    assert ent[3] < 0
+    # The reference's velocity divides by rho + 1e-100: zero density gives
+    # a finite value instead of a division by zero.
+    o.so0[0] = [0.0, 1e-300, 1.0]
+    assert o.derived('velocity')[0] == pytest.approx(1e-200)
+
+
+def test_oracle_update_cfl_closed_form(orc):
+    # Non-uniform grid: each CFL number uses the shorter of its two
+    # neighbouring intervals; negative pressure is clamped to 0, leaving
+    # only the convective speed |u|.
+    g, dt = 1.4, 0.2
+    x = np.array([0.0, 1.0, 1.5, 3.5, 4.0, 4.25, 6.0, 7.0, 7.5])
+    o = orc.Euler1D(9, dt)
+    o.coord[:] = x
+    o.so0[:] = [1.0, 0.0, 2.5]
+    o.so0[2] = [2.0, 4.0, 10.0]      # u = 2, p = 2.4
+    o.so0[3] = [0.5, -1.0, 4.0]      # u = -2, p = 1.2
+    o.so0[4] = [4.0, 2.0, 0.125]     # p < 0 -> speed |u| = 0.5
+    o.so0[5] = [1.0, 1.0, 0.5]       # p = 0 exactly -> speed 1
+    rho, m, e = o.so0.T
+    p = (g - 1) * (e - m * m / (2 * rho))
+    speed = np.sqrt(g * np.maximum(p, 0) / rho) + np.abs(m) / rho
+    dx = np.minimum(np.diff(x)[1:], np.diff(x)[:-1])     # rows 1..7
+    want = np.full(9, -7.0)
+    for odd, rows in ((False, [2, 4]), (True, [1, 3, 5])):
+        o.cfl[:] = -7.0
+        o.update_cfl(odd)
+        got = o.cfl.copy()
+        exp = want.copy()
+        exp[rows] = dt / 2 * speed[rows] / dx[np.array(rows) - 1]
+        np.testing.assert_allclose(got, exp, rtol=1e-15)
+    assert exp[5] == dt / 2 * 1.0 / 0.25
+    # The clamp: row 4's value uses |u| alone.
+    o.cfl[:] = 0
+    o.update_cfl(False)
+    assert o.cfl[4] == pytest.approx(dt / 2 * 0.5 / 0.25, rel=1e-15)
 
 
 def _periodic_solver(orc, resolution=8):
@@ -97,11 +207,9 @@ def test_oracle_fine_equals_march(orc):
 def _l1(orc, ncoord, steps):
     o = orc.sod_setup(ncoord=ncoord, time_increment=0.4 / steps)
     o.march(2, steps)
-    st = _shocktube()
     idx = np.arange(2, ncoord - 2, 2)
-    st.build_field(t=0.4, coord=o.coord[idx])
-    rho = o.so0[idx, 0]
-    return np.abs(rho - st.density_field).mean()
+    rho_exact = RiemannSolution(*SOD).sample(o.coord[idx], 0.4)[0]
+    return np.abs(o.so0[idx, 0] - rho_exact).mean()
 
 
 def test_oracle_sod_against_analytic(orc):
@@ -131,20 +239,16 @@ def test_even_ncoord_raises_in_oracle(orc):
 # ----------------------------------------------------------------- GPU -----
 
 def _gpu_sod(ncoord, steps):
-    from solvcon_amd.onedim import ShockTube
-    st = _shocktube()
-    st.build_numerical(xmin=-1, xmax=1, ncoord=ncoord,
-                       time_increment=0.4 / steps)
-    return st
+    from solvcon_amd.gen import sod_tube
+    return sod_tube(ncoord=ncoord, time_increment=0.4 / steps)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('alpha', [1, 2])
 def test_gpu_sod_bitwise(orc, alpha):
     print('case: Sod 2001 points, 600 steps, alpha', alpha, flush=True)
-    st = _gpu_sod(2001, 600)
+    svr = _gpu_sod(2001, 600)
     o = orc.sod_setup(ncoord=2001, time_increment=0.4 / 600)
-    svr = st.svr
     assert (svr.coord == o.coord).all() and (svr.so0 == o.so0).all()
     assert (svr.cfl == o.cfl).all()
     getattr(svr, 'march_alpha%d' % alpha)(600)
@@ -165,14 +269,14 @@ def test_gpu_large_bitwise(orc, ncoord):
     print('case: Sod', ncoord, 'points, 40 steps', flush=True)
     steps = 40
     dt = 0.4 / 600 * 2000 / (ncoord - 1)
-    st = _gpu_sod(ncoord, 0.4 / dt)
-    o = orc.sod_setup(ncoord=ncoord, time_increment=st.svr.time_increment)
-    st.svr.march_alpha2(steps)
+    svr = _gpu_sod(ncoord, 0.4 / dt)
+    o = orc.sod_setup(ncoord=ncoord, time_increment=svr.time_increment)
+    svr.march_alpha2(steps)
     o.march(2, steps)
     assert np.isfinite(o.so0).all() and o.cfl.max() < 0.8
-    assert (st.svr.so0 == o.so0).all()
-    assert (st.svr.so1 == o.so1).all()
-    assert (st.svr.cfl == o.cfl).all()
+    assert (svr.so0 == o.so0).all()
+    assert (svr.so1 == o.so1).all()
+    assert (svr.cfl == o.cfl).all()
 
 
 @pytest.mark.gpu
@@ -189,23 +293,28 @@ def test_gpu_tiled_bitwise(orc, monkeypatch, ncoord, steps, alpha, tiled):
           alpha, flush=True)
     monkeypatch.setenv('SC_1D_TILED', str(tiled))
     dt = 0.4 / 600 * 2000 / (ncoord - 1)
-    st = _gpu_sod(ncoord, 0.4 / dt)
-    o = orc.sod_setup(ncoord=ncoord, time_increment=st.svr.time_increment)
-    getattr(st.svr, 'march_alpha%d' % alpha)(steps)
+    svr = _gpu_sod(ncoord, 0.4 / dt)
+    o = orc.sod_setup(ncoord=ncoord, time_increment=svr.time_increment)
+    getattr(svr, 'march_alpha%d' % alpha)(steps)
     o.march(alpha, steps)
     assert np.isfinite(o.so0).all()
-    assert (st.svr.so0 == o.so0).all()
-    assert (st.svr.so1 == o.so1).all()
-    assert (st.svr.cfl == o.cfl).all()
+    assert (svr.so0 == o.so0).all()
+    assert (svr.so1 == o.so1).all()
+    assert (svr.cfl == o.cfl).all()
 
 
 @pytest.mark.gpu
 def test_gpu_fine_grained_api(orc):
     print('case: fine-grained 1D API', flush=True)
-    from solvcon_amd.onedim import Euler1DSolver
+    from solvcon_amd.onedim import Euler1DCore
     nstep, o = _periodic_solver(orc)
-    svr = Euler1DSolver(xmin=0.0, xmax=2 * np.pi, ncoord=9,
-                        time_increment=o.time_increment)
+    # Set up the way solvcon's Euler1DSolver.init_solver drives the core.
+    svr = Euler1DCore(ncoord=9, time_increment=o.time_increment)
+    svr.coord[...] = np.linspace(0.0, 2 * np.pi, num=9)
+    svr.cfl.fill(0)
+    svr.gamma.fill(1.4)
+    svr.so0[...] = 0.0
+    svr.so1[...] = 0.0
     x = svr.coord.copy()
     init = np.stack([1.0 + 0.2 * np.sin(x), 0.1 * np.cos(x),
                      2.5 + 0.1 * np.sin(2 * x)], axis=1)

@@ -81,6 +81,37 @@ def test_ce_quad_hand_values(orc):
     assert_almost_equal(mag[mag > 1e-14], 0.5 * np.sqrt(2.0), decimal=12)
 
 
+def test_single_tet_ce_closed_form(orc):
+    # The reference skips its single-tetrahedron CE test for NaN in the 3D
+    # ghost geometry (tests/test_euler_multidim.py:227).  The restatement
+    # builds it finite and equal to the closed form: each ghost is the
+    # tetrahedron mirrored across one face (centroid reflected, volume
+    # 1/6); each BCE is an inner and an outer tetrahedron of volume 1/24;
+    # the CCE is their union, 1/3 = 2 clvol.
+    m = orc.make_mesh(3, np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0),
+                                   (0, 0, 1)], float),
+                      np.array([6], np.int32),
+                      np.array([[4, 0, 1, 2, 3]], np.int32))
+    ec = orc.Euler(m, 0.01)
+    g = m.ngstcell
+    assert (g, m.ngstnode, m.ngstface) == (4, 4, 12)
+    for t in ('cevol', 'cecnd', 'sfcnd', 'sfnml'):
+        assert np.isfinite(getattr(ec, t)).all(), t
+    assert_almost_equal(m.clvol[:g], [1 / 6] * 4, decimal=15)
+    c = np.full(3, 0.25)
+    n = m.fcnml[m.ngstface:]
+    p = m.fccnd[m.ngstface:]
+    for k in range(4):
+        # Ghost -1-k mirrors the cell across boundary face bndfcs[k].
+        f = m.bndfcs[k, 0]
+        want = c - 2 * np.dot(c - p[f], n[f]) * n[f]
+        assert_almost_equal(m.clcnd[g - 1 - k], want, decimal=15)
+    assert_almost_equal(ec.cevol[g, 1:5], [1 / 12] * 4, decimal=15)
+    assert (ec.cevol[g, 5:] == 0).all()
+    assert_almost_equal(ec.cevol[g, 0], 1 / 3, decimal=15)
+    assert_almost_equal(ec.cevol[g, 1:5].sum(), ec.cevol[g, 0], decimal=15)
+
+
 @pytest.mark.parametrize('name', ALL)
 def test_bce_sum_and_total(orc, name):
     m, ec = build(orc, name)
