@@ -104,6 +104,9 @@ int scmesh_table_info(void * h, int id, int64_t * info);
 int scmesh_table_ptr(void * h, int id, void ** ptr);
 /* Host <-> device copy of a whole table (nbytes must match). */
 int scmesh_download(void * h, int id, void * host, int64_t nbytes);
+/* Device bytes of the mesh's tables, ghost rows included (no reference
+ * counterpart; admission by actual footprint in the request loop). */
+int scmesh_device_bytes(void * h, int64_t * out);
 int scmesh_upload(void * h, int id, void const * host, int64_t nbytes);
 /* StaticMesh::build_interior, StaticMesh.hpp:363-374. */
 int scmesh_build_interior(void * h, int do_metric, int do_edge);
@@ -163,6 +166,8 @@ int sceuler_sync(void * h);
 /* *done = 1 when the handle's stream is idle, else 0 (no reference
  * counterpart). */
 int sceuler_query(void * h, int * done);
+/* Device bytes of the solver's own tables (no reference counterpart). */
+int sceuler_device_bytes(void * h, int64_t * out);
 /* dt that puts the largest calc_cfl number of the current so0n at cfl,
  * reduced on the device; sceuler_set_param(h, 3, dt) then applies it (no
  * reference counterpart: the reference takes dt from its caller). */
@@ -215,6 +220,58 @@ int scge_build(void * mesh, double const * cecnd, int64_t nrow, int64_t ncol, in
 /* GradientElement::solve_gradient on a row-major dst[3][3],
  * GradientElement.cpp:447-458. */
 int scge_solve(void * mesh, int ndim, double const * dst9, double const * udf3, double * out3);
+
+/* ------------------------------------------- request loop (serving) ----- */
+/* No reference counterpart.  The reference runs one EulerCore::march
+ * (euler_march.cpp:350-359) or Euler1DCore::march_alpha<2>
+ * (Euler1DCore.hpp:339-349) per problem; a request loop calls these to
+ * march many independent problems together, each with the same arithmetic
+ * and results as its plain march.
+ *
+ * scserve_*: a resident arena of one dimension (2 or 3) with fixed
+ * capacities: type_cap[t] cells of cell type t (t = 0..8, StaticMesh type
+ * ids; 0 for none), ghost_cap ghost rows and max_slots solves.  Each
+ * substep is one fused launch per cell type over the arena. */
+int scserve_create(int ndim, int64_t const * type_cap, int64_t ghost_cap, int64_t max_slots, void ** out);
+void scserve_destroy(void * a);
+/* Admit a set-up solver (sceuler_create, then its state, boundary
+ * conditions and dt) for `steps` whole steps: its tables are copied into
+ * free ranges of the arena at the next step boundary.  *slot is -1 when
+ * its ranges or a slot do not fit now.  The solver may be destroyed after
+ * the call (its stream waits for the copies). */
+int scserve_admit(void * a, void * euler, int64_t steps, int64_t * slot);
+/* Enqueue `steps` whole steps of every admitted solve with steps left; a
+ * solve stops after its own last step and its so0n / so1n are drained to
+ * pinned host memory.  Returns without waiting. */
+int scserve_step(void * a, int64_t steps);
+/* Chunks of steps enqueued and not yet completed. */
+int scserve_pending(void * a, int64_t * chunks);
+/* *done = 1 when the slot's solve has finished and been drained. */
+int scserve_query(void * a, int64_t slot, int * done);
+/* so0n [ncell][ndim + 2] and so1n [ncell][ndim + 2][ndim] of a finished
+ * solve in its mesh's cell order (either may be NULL); frees the slot. */
+int scserve_result(void * a, int64_t slot, double * so0, double * so1);
+int scserve_sync(void * a);
+/* out[8]: used slots, marching slots, device bytes, cell capacity, ghost
+ * capacity, free cells, free ghost rows, slot capacity. */
+int scserve_info(void * a, int64_t * out);
+
+/* scserve1d_*: the 1D engine.  Admitted tubes (sc1d_create handles, set
+ * up) march together, one tiled launch per round for all of them, each
+ * with its own alpha, dt and step count. */
+int scserve1d_create(int64_t max_slots, void ** out);
+void scserve1d_destroy(void * a);
+/* Take over a set-up tube for `steps` steps of march_alpha<alpha>: on
+ * success the engine owns the handle (do not destroy it); *slot is -1
+ * when every slot is in use. */
+int scserve1d_admit(void * a, void * core, int alpha, int64_t steps, int64_t * slot);
+/* One round: each marching tube advances up to its tiled step count. */
+int scserve1d_step(void * a);
+int scserve1d_pending(void * a, int64_t * chunks);
+int scserve1d_query(void * a, int64_t slot, int * done);
+/* so0 [n*3], so1 [n*3] and cfl [n] of a finished tube (any may be NULL);
+ * frees the slot and the tube. */
+int scserve1d_result(void * a, int64_t slot, double * so0, double * so1, double * cfl);
 
 /* --------------------------------------------- spacetime scalar solvers -- */
 /* Array ids for scst_*: 0 xcoord[nx], 1 so0[nx*nvar], 2 so1[nx*nvar],

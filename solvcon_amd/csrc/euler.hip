@@ -41,6 +41,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <cmath>
+#include <deque>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -276,6 +278,18 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 
     int64_t const i0 = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x;
     int64_t const i = i0 + threadIdx.x;
+    // Batched launches may cover cells of no marching instance: the serving
+    // arena's free blocks, finished instances and the tail of an instance's
+    // last block all carry an instance whose half time increment is
+    // negative.  An instance owns whole blocks, so a block whose first cell
+    // is dead returns at once, before any load.
+    if constexpr ((MODE & M_BATCH) != 0)
+    {
+        if (p.bhdt[p.inst[g.off + i0]] < 0.0)
+        {
+            return;
+        }
+    }
 
     // The block's own rows of the current level (so0c, so1c, so0t, gamma,
     // solution points) are staged in LDS with coalesced loads; a neighbour
@@ -310,7 +324,14 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     if constexpr (GDMA)
     {
         int64_t const c = i0 + (threadIdx.x & ~63u) + 2 * (lane & 31);
-        if (c < g.cnt)
+        bool live = c < g.cnt;
+        if constexpr ((MODE & M_BATCH) != 0)
+        {
+            // No loads for the dead tail of an instance's last block, so a
+            // wave of tail cells issues none and may return early.
+            live = live && p.bhdt[p.inst[g.off + c]] >= 0.0;
+        }
+        if (live)
         {
 #pragma unroll
             for (int q = 0; q < GN / 2; ++q)
@@ -356,6 +377,13 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     if (i >= g.cnt)
     {
         return;
+    }
+    if constexpr ((MODE & M_BATCH) != 0)
+    {
+        if (p.bhdt[p.inst[g.off + i]] < 0.0)
+        {
+            return;
+        }
     }
     int64_t const n = g.n;
     int64_t const row = p.G + g.off + i;
@@ -3432,8 +3460,813 @@ void DEuler::copy(int id, double * host, int64_t count, bool down)
     sync();
 }
 
+// --------------------------------------------------------------------------
+// The serving arena: continuous batching of independent solves.
+//
+// One arena per dimension holds, at fixed capacity, the tables the batched
+// fused kernel reads: per cell type a group of SoA geometry and neighbour
+// lists, and the row tables (ghost rows first, then each type group).  A
+// request is set up as a plain EulerCore of its own mesh (device mesh
+// build, CE geometry, state, boundary conditions, dt) and admitted by
+// copying its tables into free whole-block ranges of the arena, with its
+// neighbour rows renumbered.  Every substep is one fused launch per type
+// group over the arena's used prefix; blocks of free ranges and of
+// finished solves return at once (their instance's half time increment is
+// negative).  A solve that has run its steps is marked finished and its
+// state drained to pinned host memory on the arena's stream; its ranges
+// are reused after its result is taken.  Each solve's arithmetic is that
+// of the plain march: its cells read only its own rows.
+// --------------------------------------------------------------------------
+
+namespace
+{
+
+// First-fit allocator of [0, cap) in whole ranges; merges on release.
+struct RangeAlloc
+{
+    int64_t cap = 0;
+    std::map<int64_t, int64_t> free_; // start -> end
+
+    void init(int64_t c)
+    {
+        cap = c;
+        free_.clear();
+        if (c > 0)
+        {
+            free_[0] = c;
+        }
+    }
+    int64_t take(int64_t n)
+    {
+        if (n <= 0)
+        {
+            return 0;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+        {
+            if (it->second - it->first >= n)
+            {
+                int64_t const a = it->first;
+                int64_t const e = it->second;
+                free_.erase(it);
+                if (a + n < e)
+                {
+                    free_[a + n] = e;
+                }
+                return a;
+            }
+        }
+        return -1;
+    }
+    void give(int64_t a, int64_t n)
+    {
+        if (n <= 0)
+        {
+            return;
+        }
+        int64_t e = a + n;
+        auto it = free_.lower_bound(a);
+        if (it != free_.end() && it->first == e)
+        {
+            e = it->second;
+            it = free_.erase(it);
+        }
+        if (it != free_.begin())
+        {
+            auto pv = std::prev(it);
+            if (pv->second == a)
+            {
+                a = pv->first;
+                free_.erase(pv);
+            }
+        }
+        free_[a] = e;
+    }
+    // End of the used prefix.
+    int64_t hwm() const
+    {
+        if (free_.empty())
+        {
+            return cap;
+        }
+        auto last = std::prev(free_.end());
+        return last->second == cap ? last->first : cap;
+    }
+    int64_t free_total() const
+    {
+        int64_t t = 0;
+        for (auto const & kv : free_)
+        {
+            t += kv.second - kv.first;
+        }
+        return t;
+    }
+};
+
+// Renumber an instance's neighbour rows into the arena: ghost row r to
+// ghost0 + r, body row G + b of instance group k (its cells
+// [goff[k], goff[k] + gn[k]) of the instance's body order) to
+// abase[k] + b - goff[k].
+struct RowMap
+{
+    int64_t G;
+    int64_t ghost0;
+    int ng;
+    int64_t goff[4];
+    int64_t gn[4];
+    int64_t abase[4];
+};
+
+__device__ inline int64_t map_row(RowMap const & m, int64_t r)
+{
+    if (r < m.G)
+    {
+        return m.ghost0 + r;
+    }
+    int64_t const b = r - m.G;
+    int64_t out = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        if (k < m.ng && b >= m.goff[k] && b < m.goff[k] + m.gn[k])
+        {
+            out = m.abase[k] + (b - m.goff[k]);
+        }
+    }
+    return out;
+}
+
+__global__ void k_arena_nbr(int nf, int64_t n, int32_t const * src, int64_t cap, int32_t * dst, RowMap m)
+{
+    int64_t const t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nf * n)
+    {
+        return;
+    }
+    int64_t const f = t / n;
+    int64_t const i = t - f * n;
+    dst[f * cap + i] = static_cast<int32_t>(map_row(m, src[f * n + i]));
+}
+
+__global__ void k_arena_ginl(int64_t n, int32_t const * src, int32_t add, int32_t * dst)
+{
+    int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        dst[i] = src[i] + add;
+    }
+}
+
+__global__ void k_arena_set_dt(double * bhdt, double * bqdt, int64_t slot, double h, double q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        bhdt[slot] = h;
+        bqdt[slot] = q;
+    }
+}
+
+} /* end namespace */
+
+struct DArena
+{
+    int ndim = 0;
+    int neq = 0;
+    int64_t Gcap = 0;
+    int64_t Ncap = 0;
+    int64_t Rcap = 0;
+    int64_t max_slots = 0;
+    int64_t inl_cap = 0;
+    double sigma0 = 3.0;
+    double taumin = 0.0;
+    double tauscale = 1.0;
+    StreamHolder sh; // declared before the buffers: freed last
+    hipStream_t stream = sh.s;
+
+    struct AGroup
+    {
+        int32_t type = 0;
+        int64_t off = 0;
+        int64_t cap = 0;
+        int nf = 0;
+        int nfield = 0;
+        DevBuf<int32_t> nbr;
+        DevBuf<double> geo;
+        RangeAlloc ra;
+    };
+    std::vector<AGroup> groups;
+    DevBuf<double> cen, s0[2], s1[2], st[2], gam, cflo, cflc, gmat, inlet;
+    DevBuf<int32_t> gkind, ginl, binst;
+    DevBuf<double> bhdt, bqdt; // per slot; entry max_slots is the dead instance
+    RangeAlloc ghosts, inlets;
+    int cur = 0;
+
+    struct Slot
+    {
+        bool used = false;
+        bool marching = false;
+        bool done_ok = false;
+        int64_t steps_left = 0;
+        int64_t ncell = 0;
+        int64_t G = 0;
+        int64_t ghost0 = 0;
+        int64_t inl0 = 0;
+        int64_t ninl = 0;
+        std::vector<int> gidx;        // arena group of each instance group
+        std::vector<int64_t> start;   // arena cell start per instance group
+        std::vector<int64_t> count;   // cells per instance group
+        std::vector<int64_t> ioff;    // instance body offset per instance group
+        std::vector<int32_t> ref_of_int;
+        double * host = nullptr;      // pinned: so0 then so1 body rows, internal order
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Slot> slots;
+    std::deque<hipEvent_t> chunks; // events after enqueued step chunks
+
+    DArena(int nd, int64_t const * type_cap, int64_t ghost_cap, int64_t nslot)
+    {
+        if (nd != 2 && nd != 3)
+        {
+            fail(SC_EVALUE, "serve arena: ndim must be 2 or 3");
+        }
+        if (nslot < 1 || nslot > (1 << 20))
+        {
+            fail(SC_EVALUE, "serve arena: between 1 and 1048576 slots");
+        }
+        if (ghost_cap < 0)
+        {
+            fail(SC_EVALUE, "serve arena: negative ghost capacity");
+        }
+        AllocScope const scope(stream);
+        ndim = nd;
+        neq = nd + 2;
+        max_slots = nslot;
+        inl_cap = 64 * nslot;
+        int const types2[] = {CT_QUAD, CT_TRI};
+        int const types3[] = {CT_HEX, CT_TET, CT_PRISM, CT_PYRAMID};
+        int const * tl = nd == 2 ? types2 : types3;
+        int const nt = nd == 2 ? 2 : 4;
+        int64_t off = 0;
+        for (int k = 0; k < nt; ++k)
+        {
+            int const t = tl[k];
+            int64_t c = type_cap[t];
+            if (c < 0)
+            {
+                fail(SC_EVALUE, "serve arena: negative cell capacity");
+            }
+            if (c == 0)
+            {
+                continue;
+            }
+            c = (c + BLK - 1) / BLK * BLK;
+            AGroup g;
+            g.type = t;
+            g.off = off;
+            g.cap = c;
+            switch (t)
+            {
+            case CT_TRI: g.nf = Traits<CT_TRI>::NF; g.nfield = Layout<CT_TRI>::NFIELD; break;
+            case CT_QUAD: g.nf = Traits<CT_QUAD>::NF; g.nfield = Layout<CT_QUAD>::NFIELD; break;
+            case CT_TET: g.nf = Traits<CT_TET>::NF; g.nfield = Layout<CT_TET>::NFIELD; break;
+            case CT_HEX: g.nf = Traits<CT_HEX>::NF; g.nfield = Layout<CT_HEX>::NFIELD; break;
+            case CT_PRISM: g.nf = Traits<CT_PRISM>::NF; g.nfield = Layout<CT_PRISM>::NFIELD; break;
+            case CT_PYRAMID: g.nf = Traits<CT_PYRAMID>::NF; g.nfield = Layout<CT_PYRAMID>::NFIELD; break;
+            default: break;
+            }
+            g.nbr.alloc(static_cast<int64_t>(g.nf) * c);
+            g.geo.alloc(static_cast<int64_t>(g.nfield) * c);
+            // Finite fill: dead blocks never read it, but the geometry
+            // prefetch of a live block's tail may load it.
+            SC_HIP(hipMemsetAsync(g.geo.ptr, 0, g.geo.bytes(), stream));
+            fill<int32_t>(g.nbr.ptr, static_cast<int64_t>(g.nf) * c, 0, stream);
+            g.ra.init(c);
+            off += c;
+            groups.push_back(std::move(g));
+        }
+        Ncap = off;
+        Gcap = ghost_cap;
+        Rcap = Gcap + Ncap;
+        cen.alloc(static_cast<int64_t>(ndim) * Rcap);
+        for (int b = 0; b < 2; ++b)
+        {
+            s0[b].alloc(static_cast<int64_t>(neq) * Rcap);
+            s1[b].alloc(static_cast<int64_t>(neq) * ndim * Rcap);
+            st[b].alloc(static_cast<int64_t>(neq) * Rcap);
+        }
+        gam.alloc(Rcap);
+        cflo.alloc(Rcap);
+        cflc.alloc(Rcap);
+        for (auto * b : {&s0[0], &s0[1], &s1[0], &s1[1], &st[0], &st[1], &gam, &cflo, &cflc, &cen})
+        {
+            if (b->count)
+            {
+                SC_HIP(hipMemsetAsync(b->ptr, 0, b->bytes(), stream));
+            }
+        }
+        gkind.alloc(Gcap);
+        ginl.alloc(Gcap);
+        gmat.alloc(static_cast<int64_t>(ndim) * ndim * Gcap);
+        fill<int32_t>(gkind.ptr, Gcap, BC_NONE, stream);
+        fill<int32_t>(ginl.ptr, Gcap, 0, stream);
+        inlet.alloc(inl_cap * (ndim + 3));
+        binst.alloc(Ncap);
+        fill<int32_t>(binst.ptr, Ncap, static_cast<int32_t>(max_slots), stream);
+        bhdt.alloc(max_slots + 1);
+        bqdt.alloc(max_slots + 1);
+        fill<double>(bhdt.ptr, max_slots + 1, -1.0, stream);
+        fill<double>(bqdt.ptr, max_slots + 1, -1.0, stream);
+        ghosts.init(Gcap);
+        inlets.init(inl_cap);
+        slots.resize(static_cast<size_t>(max_slots));
+        sync();
+    }
+
+    ~DArena()
+    {
+        (void)hipStreamSynchronize(stream);
+        for (auto ev : chunks)
+        {
+            (void)hipEventDestroy(ev);
+        }
+        for (auto & sl : slots)
+        {
+            release_host(sl);
+        }
+    }
+
+    void sync() { SC_HIP(hipStreamSynchronize(stream)); }
+
+    static void release_host(Slot & sl)
+    {
+        if (sl.done != nullptr)
+        {
+            (void)hipEventDestroy(sl.done);
+            sl.done = nullptr;
+        }
+        if (sl.host != nullptr)
+        {
+            (void)hipHostFree(sl.host);
+            sl.host = nullptr;
+        }
+    }
+
+    int group_of(int32_t type) const
+    {
+        for (size_t k = 0; k < groups.size(); ++k)
+        {
+            if (groups[k].type == type)
+            {
+                return static_cast<int>(k);
+            }
+        }
+        return -1;
+    }
+
+    KP params() const
+    {
+        KP p;
+        p.R = Rcap;
+        p.G = Gcap;
+        p.hdt = 0.0;
+        p.qdt = 0.0;
+        p.sigma0 = sigma0;
+        p.taumin = taumin;
+        p.tauscale = tauscale;
+        p.s0c = s0[cur].ptr;
+        p.s1c = s1[cur].ptr;
+        p.s0n = s0[1 - cur].ptr;
+        p.s1n = s1[1 - cur].ptr;
+        p.st = st[cur].ptr;
+        p.stn = st[1 - cur].ptr;
+        p.gam = gam.ptr;
+        p.cflo = cflo.ptr;
+        p.cflc = cflc.ptr;
+        p.cen = cen.ptr;
+        p.gkind = gkind.ptr;
+        p.ginl = ginl.ptr;
+        p.gmat = gmat.ptr;
+        p.inlet = inlet.ptr;
+        p.inst = binst.ptr;
+        p.bhdt = bhdt.ptr;
+        p.bqdt = bqdt.ptr;
+        return p;
+    }
+
+    // Admit a set-up solver for `steps` whole steps; returns the slot, or
+    // -1 when its ranges do not fit now.  The solver's tables are copied on
+    // the arena's stream after the solver's stream work; the solver's
+    // stream then waits for the copies, so it may be destroyed at once.
+    int64_t admit(DEuler & e, int64_t steps)
+    {
+        if (e.ninst != 0)
+        {
+            fail(SC_EVALUE, "serve arena: a batched solver cannot be admitted");
+        }
+        if (e.ndim != ndim)
+        {
+            fail(SC_EVALUE, "serve arena: the solver's ndim differs from the arena's");
+        }
+        if (steps < 1)
+        {
+            fail(SC_EVALUE, "serve arena: a solve needs at least one step");
+        }
+        if (e.sigma0 != sigma0 || e.taumin != taumin || e.tauscale != tauscale)
+        {
+            fail(SC_EVALUE, "serve arena: the solver's sigma0/taumin/tauscale differ from the arena's");
+        }
+        if (e.groups.size() > 4)
+        {
+            fail(SC_ERUNTIME, "serve arena: too many cell-type groups");
+        }
+        int64_t slot = -1;
+        for (int64_t k = 0; k < max_slots; ++k)
+        {
+            if (!slots[k].used)
+            {
+                slot = k;
+                break;
+            }
+        }
+        if (slot < 0)
+        {
+            return -1;
+        }
+        int64_t ninl = 0;
+        for (auto const & b : e.bcs)
+        {
+            if (b.kind == BC_INLET)
+            {
+                ++ninl;
+            }
+        }
+        // Take the ranges, or give back what was taken.
+        std::vector<int> gidx;
+        std::vector<int64_t> start;
+        for (auto const & g : e.groups)
+        {
+            int const a = group_of(g.type);
+            if (a < 0)
+            {
+                fail(SC_EVALUE, "serve arena: no capacity for cell type " + std::to_string(g.type));
+            }
+            gidx.push_back(a);
+        }
+        bool ok = true;
+        for (size_t k = 0; k < e.groups.size() && ok; ++k)
+        {
+            int64_t const nb = (e.groups[k].n + BLK - 1) / BLK * BLK;
+            int64_t const a = groups[gidx[k]].ra.take(nb);
+            if (a < 0)
+            {
+                ok = false;
+            }
+            else
+            {
+                start.push_back(a);
+            }
+        }
+        int64_t const g0 = ok ? ghosts.take(e.G) : -1;
+        ok = ok && g0 >= 0;
+        int64_t const i0 = ok ? inlets.take(ninl) : -1;
+        ok = ok && i0 >= 0;
+        if (!ok)
+        {
+            for (size_t k = 0; k < start.size(); ++k)
+            {
+                groups[gidx[k]].ra.give(start[k], (e.groups[k].n + BLK - 1) / BLK * BLK);
+            }
+            if (g0 >= 0)
+            {
+                ghosts.give(g0, e.G);
+            }
+            return -1;
+        }
+        AllocScope const scope(stream);
+        Slot & sl = slots[slot];
+        sl = Slot{};
+        sl.used = true;
+        sl.marching = true;
+        sl.steps_left = steps;
+        sl.ncell = e.N;
+        sl.G = e.G;
+        sl.ghost0 = g0;
+        sl.inl0 = i0;
+        sl.ninl = ninl;
+        sl.gidx = gidx;
+        sl.start = start;
+        for (auto const & g : e.groups)
+        {
+            sl.count.push_back(g.n);
+            sl.ioff.push_back(g.off);
+        }
+        try
+        {
+            copy_in(e, sl, slot);
+        }
+        catch (...)
+        {
+            free_slot(slot);
+            throw;
+        }
+        return slot;
+    }
+
+    void copy_in(DEuler & e, Slot & sl, int64_t slot)
+    {
+        hipStream_t const s = stream;
+        int const nd = ndim;
+        int const q = neq;
+        // The instance's reference order of its body cells.
+        sl.ref_of_int.resize(static_cast<size_t>(e.N));
+        if (e.N > 0)
+        {
+            SC_HIP(hipMemcpyAsync(sl.ref_of_int.data(), e.ref_of_int.ptr, e.ref_of_int.bytes(), hipMemcpyDeviceToHost,
+                                  e.stream));
+        }
+        e.sync();
+        hipEvent_t ev = nullptr;
+        SC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        RowMap m{};
+        m.G = e.G;
+        m.ghost0 = sl.ghost0;
+        m.ng = static_cast<int>(e.groups.size());
+        for (size_t k = 0; k < e.groups.size(); ++k)
+        {
+            m.goff[k] = e.groups[k].off;
+            m.gn[k] = e.groups[k].n;
+            m.abase[k] = Gcap + groups[sl.gidx[k]].off + sl.start[k];
+        }
+        // The instance's so0n / so1n (s*[1 - cur]) land in the arena's
+        // so0n / so1n, and its other level likewise.
+        int const bmap[2] = {e.cur == cur ? 0 : 1, e.cur == cur ? 1 : 0};
+        auto rows = [&](double const * src, double * dst, int64_t w, int64_t r_src, int64_t r_dst, int64_t n)
+        {
+            if (n > 0)
+            {
+                SC_HIP(hipMemcpyAsync(dst + r_dst * w, src + r_src * w, sizeof(double) * w * n,
+                                      hipMemcpyDeviceToDevice, s));
+            }
+        };
+        auto all_rows = [&](int64_t r_src, int64_t r_dst, int64_t n)
+        {
+            rows(e.cen.ptr, cen.ptr, nd, r_src, r_dst, n);
+            for (int b = 0; b < 2; ++b)
+            {
+                rows(e.s0[b].ptr, s0[bmap[b]].ptr, q, r_src, r_dst, n);
+                rows(e.s1[b].ptr, s1[bmap[b]].ptr, q * nd, r_src, r_dst, n);
+            }
+            rows(e.gam.ptr, gam.ptr, 1, r_src, r_dst, n);
+            rows(e.cflo.ptr, cflo.ptr, 1, r_src, r_dst, n);
+            rows(e.cflc.ptr, cflc.ptr, 1, r_src, r_dst, n);
+        };
+        try
+        {
+            for (size_t k = 0; k < e.groups.size(); ++k)
+            {
+                auto const & eg = e.groups[k];
+                auto & ag = groups[sl.gidx[k]];
+                int64_t const a = sl.start[k];
+                SC_HIP(hipMemcpy2DAsync(ag.geo.ptr + a, sizeof(double) * ag.cap, eg.geo.ptr, sizeof(double) * eg.n,
+                                        sizeof(double) * eg.n, ag.nfield, hipMemcpyDeviceToDevice, s));
+                k_arena_nbr<<<grid_for(ag.nf * eg.n, BLK), BLK, 0, s>>>(ag.nf, eg.n, eg.nbr.ptr, ag.cap,
+                                                                        ag.nbr.ptr + a, m);
+                SC_HIP(hipGetLastError());
+                all_rows(e.G + eg.off, Gcap + ag.off + a, eg.n);
+                int64_t const nb = (eg.n + BLK - 1) / BLK * BLK;
+                fill<int32_t>(binst.ptr + ag.off + a, eg.n, static_cast<int32_t>(slot), s);
+                fill<int32_t>(binst.ptr + ag.off + a + eg.n, nb - eg.n, static_cast<int32_t>(max_slots), s);
+            }
+            if (e.G > 0)
+            {
+                all_rows(0, sl.ghost0, e.G);
+                SC_HIP(hipMemcpyAsync(gkind.ptr + sl.ghost0, e.gkind.ptr, sizeof(int32_t) * e.G,
+                                      hipMemcpyDeviceToDevice, s));
+                k_arena_ginl<<<grid_for(e.G, BLK), BLK, 0, s>>>(e.G, e.ginl.ptr, static_cast<int32_t>(sl.inl0),
+                                                                ginl.ptr + sl.ghost0);
+                SC_HIP(hipGetLastError());
+                SC_HIP(hipMemcpy2DAsync(gmat.ptr + sl.ghost0, sizeof(double) * Gcap, e.gmat.ptr, sizeof(double) * e.G,
+                                        sizeof(double) * e.G, nd * nd, hipMemcpyDeviceToDevice, s));
+            }
+            if (sl.ninl > 0)
+            {
+                SC_HIP(hipMemcpyAsync(inlet.ptr + sl.inl0 * (nd + 3), e.inlet.ptr, sizeof(double) * sl.ninl * (nd + 3),
+                                      hipMemcpyDeviceToDevice, s));
+            }
+            k_arena_set_dt<<<1, 64, 0, s>>>(bhdt.ptr, bqdt.ptr, slot, e.dt * 0.5, e.dt * 0.25);
+            SC_HIP(hipGetLastError());
+            SC_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.host), sizeof(double) * e.N * q * (1 + nd)));
+            SC_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+            // The solver's buffers are freed on its own stream: order that
+            // after the copies.
+            SC_HIP(hipEventRecord(ev, s));
+            SC_HIP(hipStreamWaitEvent(e.stream, ev, 0));
+        }
+        catch (...)
+        {
+            (void)hipEventDestroy(ev);
+            throw;
+        }
+        (void)hipEventDestroy(ev);
+    }
+
+    // Return a slot's ranges; its cells become dead first.
+    void free_slot(int64_t slot)
+    {
+        Slot & sl = slots[slot];
+        if (!sl.used)
+        {
+            return;
+        }
+        AllocScope const scope(stream);
+        k_arena_set_dt<<<1, 64, 0, stream>>>(bhdt.ptr, bqdt.ptr, slot, -1.0, -1.0);
+        SC_HIP(hipGetLastError());
+        for (size_t k = 0; k < sl.start.size(); ++k)
+        {
+            auto & ag = groups[sl.gidx[k]];
+            int64_t const nb = (sl.count[k] + BLK - 1) / BLK * BLK;
+            fill<int32_t>(binst.ptr + ag.off + sl.start[k], nb, static_cast<int32_t>(max_slots), stream);
+            ag.ra.give(sl.start[k], nb);
+        }
+        ghosts.give(sl.ghost0, sl.G);
+        inlets.give(sl.inl0, sl.ninl);
+        release_host(sl);
+        sl = Slot{};
+    }
+
+    void launch_groups()
+    {
+        KP const p = params();
+        for (auto & gr : groups)
+        {
+            int64_t const hwm = gr.ra.hwm();
+            if (hwm <= 0)
+            {
+                continue;
+            }
+            GV const g{gr.off, gr.cap, gr.nbr.ptr, gr.geo.ptr, hwm};
+            unsigned const nb = grid_for(hwm, BLK);
+            constexpr int MB = M_FUSED | M_BATCH;
+            constexpr int MT = MB | batch_waves(CT_TRI);
+            switch (gr.type)
+            {
+            case CT_TRI: k_cell<CT_TRI, MT | M_GDMA><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_QUAD: k_cell<CT_QUAD, MB | batch_waves(CT_QUAD)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TET: k_cell<CT_TET, MB | batch_waves(CT_TET)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_HEX: k_cell<CT_HEX, MB | batch_waves(CT_HEX)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PRISM: k_cell<CT_PRISM, MB | batch_waves(CT_PRISM)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PYRAMID: k_cell<CT_PYRAMID, MB | batch_waves(CT_PYRAMID)><<<nb, BLK, 0, stream>>>(p, g); break;
+            default: fail(SC_ERUNTIME, "serve arena: unsupported cell type in group");
+            }
+            SC_HIP(hipGetLastError());
+        }
+    }
+
+    // Enqueue k whole steps of every marching solve.  A solve whose steps
+    // end within them is marked finished after its last step, and its
+    // so0n and so1n body rows are drained to its pinned buffer.
+    void step(int64_t k)
+    {
+        AllocScope const scope(stream);
+        for (int64_t it = 0; it < k; ++it)
+        {
+            bool any = false;
+            for (auto const & sl : slots)
+            {
+                any = any || sl.marching;
+            }
+            if (!any)
+            {
+                break;
+            }
+            for (int sub = 0; sub < 2; ++sub)
+            {
+                cur = 1 - cur;
+                launch_groups();
+            }
+            for (int64_t slot = 0; slot < max_slots; ++slot)
+            {
+                Slot & sl = slots[slot];
+                if (!sl.marching || --sl.steps_left > 0)
+                {
+                    continue;
+                }
+                sl.marching = false;
+                k_arena_set_dt<<<1, 64, 0, stream>>>(bhdt.ptr, bqdt.ptr, slot, -1.0, -1.0);
+                SC_HIP(hipGetLastError());
+                int64_t const q = neq;
+                for (size_t g = 0; g < sl.start.size(); ++g)
+                {
+                    auto const & ag = groups[sl.gidx[g]];
+                    int64_t const r = Gcap + ag.off + sl.start[g];
+                    SC_HIP(hipMemcpyAsync(sl.host + (sl.ioff[g]) * q, s0[1 - cur].ptr + r * q,
+                                          sizeof(double) * sl.count[g] * q, hipMemcpyDeviceToHost, stream));
+                    SC_HIP(hipMemcpyAsync(sl.host + sl.ncell * q + sl.ioff[g] * q * ndim, s1[1 - cur].ptr + r * q * ndim,
+                                          sizeof(double) * sl.count[g] * q * ndim, hipMemcpyDeviceToHost, stream));
+                }
+                SC_HIP(hipEventRecord(sl.done, stream));
+            }
+        }
+        hipEvent_t ev = nullptr;
+        SC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SC_HIP(hipEventRecord(ev, stream));
+        chunks.push_back(ev);
+    }
+
+    // Enqueued step chunks not yet completed.
+    int64_t pending()
+    {
+        while (!chunks.empty())
+        {
+            hipError_t const e = hipEventQuery(chunks.front());
+            if (e == hipErrorNotReady)
+            {
+                break;
+            }
+            SC_HIP(e);
+            (void)hipEventDestroy(chunks.front());
+            chunks.pop_front();
+        }
+        return static_cast<int64_t>(chunks.size());
+    }
+
+    // 1 when the slot's result has been drained, 0 while it marches.
+    int query(int64_t slot)
+    {
+        Slot & sl = slots[slot];
+        if (!sl.used)
+        {
+            fail(SC_EVALUE, "serve arena: slot " + std::to_string(slot) + " holds no solve");
+        }
+        if (sl.marching)
+        {
+            return 0;
+        }
+        hipError_t const e = hipEventQuery(sl.done);
+        if (e == hipErrorNotReady)
+        {
+            return 0;
+        }
+        SC_HIP(e);
+        return 1;
+    }
+
+    // The drained so0n [ncell][neq] and so1n [ncell][neq][ndim] in the
+    // reference cell order (either pointer may be null); frees the slot.
+    void result(int64_t slot, double * so0_out, double * so1_out)
+    {
+        Slot & sl = slots[slot];
+        if (!sl.used)
+        {
+            fail(SC_EVALUE, "serve arena: slot " + std::to_string(slot) + " holds no solve");
+        }
+        if (sl.marching)
+        {
+            fail(SC_ERUNTIME, "serve arena: slot " + std::to_string(slot) + " is still marching");
+        }
+        SC_HIP(hipEventSynchronize(sl.done));
+        int64_t const q = neq;
+        int64_t const w1 = q * ndim;
+        double const * h0 = sl.host;
+        double const * h1 = sl.host + sl.ncell * q;
+        for (int64_t c = 0; c < sl.ncell; ++c)
+        {
+            int64_t const r = sl.ref_of_int[c];
+            if (so0_out != nullptr)
+            {
+                std::memcpy(so0_out + r * q, h0 + c * q, sizeof(double) * q);
+            }
+            if (so1_out != nullptr)
+            {
+                std::memcpy(so1_out + r * w1, h1 + c * w1, sizeof(double) * w1);
+            }
+        }
+        free_slot(slot);
+    }
+
+    int64_t device_bytes() const
+    {
+        int64_t b = 0;
+        for (auto const & g : groups)
+        {
+            b += static_cast<int64_t>(g.nbr.bytes() + g.geo.bytes());
+        }
+        for (auto const * x : {&cen, &s0[0], &s0[1], &s1[0], &s1[1], &st[0], &st[1], &gam, &cflo, &cflc, &gmat, &inlet,
+                               &bhdt, &bqdt})
+        {
+            b += static_cast<int64_t>(x->bytes());
+        }
+        for (auto const * x : {&gkind, &ginl, &binst})
+        {
+            b += static_cast<int64_t>(x->bytes());
+        }
+        return b;
+    }
+};
+
 } /* end namespace scamd */
 
+using scamd::DArena;
 using scamd::DEuler;
 using scamd::DMesh;
 using scamd::guard;
@@ -3927,6 +4760,127 @@ int sceuler_reduce(void * h, double * out)
             out[0] = s;
             out[1] = lo;
             out[2] = hi;
+        });
+}
+
+// Device bytes a solver holds (its own tables; the mesh's are counted by
+// scmesh_device_bytes).
+int sceuler_device_bytes(void * h, int64_t * out)
+{
+    return guard(
+        [&]()
+        {
+            auto & e = *static_cast<DEuler *>(h);
+            int64_t b = 0;
+            for (auto const & g : e.groups)
+            {
+                b += static_cast<int64_t>(g.nbr.bytes() + g.geo.bytes());
+            }
+            for (auto const * x : {&e.cen, &e.s0[0], &e.s0[1], &e.s1[0], &e.s1[1], &e.st[0], &e.st[1], &e.stm, &e.gam,
+                                   &e.cflo, &e.cflc, &e.gmat, &e.inlet, &e.bhdt, &e.bqdt})
+            {
+                b += static_cast<int64_t>(x->bytes());
+            }
+            for (auto const * x : {&e.int_of_ref, &e.ref_of_int, &e.gkind, &e.ginl, &e.gsrc, &e.brank_ref, &e.binst})
+            {
+                b += static_cast<int64_t>(x->bytes());
+            }
+            *out = b;
+        });
+}
+
+int scserve_create(int ndim, int64_t const * type_cap, int64_t ghost_cap, int64_t max_slots, void ** out)
+{
+    return guard([&]() { *out = new DArena(ndim, type_cap, ghost_cap, max_slots); });
+}
+
+void scserve_destroy(void * a) { delete static_cast<DArena *>(a); }
+
+int scserve_admit(void * a, void * euler, int64_t steps, int64_t * slot)
+{
+    return guard([&]() { *slot = static_cast<DArena *>(a)->admit(*static_cast<DEuler *>(euler), steps); });
+}
+
+int scserve_step(void * a, int64_t steps)
+{
+    return guard(
+        [&]()
+        {
+            if (steps < 0)
+            {
+                scamd::fail(scamd::SC_EVALUE, "serve arena: steps must be >= 0");
+            }
+            static_cast<DArena *>(a)->step(steps);
+        });
+}
+
+int scserve_pending(void * a, int64_t * chunks)
+{
+    return guard([&]() { *chunks = static_cast<DArena *>(a)->pending(); });
+}
+
+int scserve_query(void * a, int64_t slot, int * done)
+{
+    return guard(
+        [&]()
+        {
+            auto & ar = *static_cast<DArena *>(a);
+            if (slot < 0 || slot >= ar.max_slots)
+            {
+                scamd::fail(scamd::SC_EINDEX, "serve arena: slot out of range");
+            }
+            *done = ar.query(slot);
+        });
+}
+
+int scserve_result(void * a, int64_t slot, double * so0, double * so1)
+{
+    return guard(
+        [&]()
+        {
+            auto & ar = *static_cast<DArena *>(a);
+            if (slot < 0 || slot >= ar.max_slots)
+            {
+                scamd::fail(scamd::SC_EINDEX, "serve arena: slot out of range");
+            }
+            ar.result(slot, so0, so1);
+        });
+}
+
+int scserve_sync(void * a)
+{
+    return guard([&]() { static_cast<DArena *>(a)->sync(); });
+}
+
+// out[0] used slots, [1] marching slots, [2] device bytes, [3] cell
+// capacity, [4] ghost capacity, [5] free cells, [6] free ghost rows,
+// [7] slot capacity.
+int scserve_info(void * a, int64_t * out)
+{
+    return guard(
+        [&]()
+        {
+            auto & ar = *static_cast<DArena *>(a);
+            int64_t used = 0;
+            int64_t marching = 0;
+            for (auto const & sl : ar.slots)
+            {
+                used += sl.used ? 1 : 0;
+                marching += sl.marching ? 1 : 0;
+            }
+            int64_t fc = 0;
+            for (auto const & g : ar.groups)
+            {
+                fc += g.ra.free_total();
+            }
+            out[0] = used;
+            out[1] = marching;
+            out[2] = ar.device_bytes();
+            out[3] = ar.Ncap;
+            out[4] = ar.Gcap;
+            out[5] = fc;
+            out[6] = ar.ghosts.free_total();
+            out[7] = ar.max_slots;
         });
 }
 

@@ -1560,4 +1560,25 @@ int scmesh_bc(void * h, int ibc, char * name, int nname, int32_t * facn0, int32_
         });
 }
 
+// Device bytes a mesh holds (its tables, ghost rows included).
+int scmesh_device_bytes(void * h, int64_t * out)
+{
+    return guard(
+        [&]()
+        {
+            auto & m = *static_cast<DMesh *>(h);
+            int64_t b = 0;
+            for (auto const * x : {&m.ndcrd, &m.fccnd, &m.fcnml, &m.fcara, &m.clcnd, &m.clvol})
+            {
+                b += static_cast<int64_t>(x->bytes());
+            }
+            for (auto const * x : {&m.fctpn, &m.cltpn, &m.clgrp, &m.fcnds, &m.fccls, &m.clnds, &m.clfcs, &m.ednds,
+                                   &m.bndfcs})
+            {
+                b += static_cast<int64_t>(x->bytes());
+            }
+            *out = b;
+        });
+}
+
 } /* end extern "C" */

@@ -23,6 +23,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <memory>
+#include <cstring>
+#include <deque>
+#include <vector>
 #include <utility>
 
 namespace scamd
@@ -380,14 +383,12 @@ constexpr int TBLOCK = 512;
 constexpr int TROWS = 1024; // most rows of one window (48 KB of LDS)
 
 template <int ALPHA>
-__global__ void __launch_bounds__(TBLOCK) k_march_tiled(int64_t n, int64_t seg, int64_t halo, int64_t steps, int last,
-                                                        double hdt, double qdt, double const * coord,
-                                                        double const * in0, double const * in1, double * out0,
-                                                        double * out1, double const * gamma, double * cfl)
+__device__ inline void tiled_segment(int64_t blk, double * s0, double * s1, int64_t n, int64_t seg, int64_t halo,
+                                     int64_t steps, int last, double hdt, double qdt, double const * coord,
+                                     double const * in0, double const * in1, double * out0, double * out1,
+                                     double const * gamma, double * cfl)
 {
-    __shared__ double s0[TROWS * 3];
-    __shared__ double s1[TROWS * 3];
-    int64_t const own0 = static_cast<int64_t>(blockIdx.x) * seg;
+    int64_t const own0 = blk * seg;
     int64_t const own1 = own0 + seg < n ? own0 + seg : n;
     int64_t const lo = own0 - halo > 0 ? own0 - halo : 0;
     int64_t const hi = own1 + halo < n ? own1 + halo : n;
@@ -466,6 +467,65 @@ __global__ void __launch_bounds__(TBLOCK) k_march_tiled(int64_t n, int64_t seg, 
     }
 }
 
+
+template <int ALPHA>
+__global__ void __launch_bounds__(TBLOCK) k_march_tiled(int64_t n, int64_t seg, int64_t halo, int64_t steps, int last,
+                                                        double hdt, double qdt, double const * coord,
+                                                        double const * in0, double const * in1, double * out0,
+                                                        double * out1, double const * gamma, double * cfl)
+{
+    __shared__ double s0[TROWS * 3];
+    __shared__ double s1[TROWS * 3];
+    tiled_segment<ALPHA>(blockIdx.x, s0, s1, n, seg, halo, steps, last, hdt, qdt, coord, in0, in1, out0, out1, gamma,
+                         cfl);
+}
+
+// Several tubes in one launch (the request loop's batched 1D march): each
+// tube's blocks are a contiguous range of the grid, and each tube runs its
+// own step count.  The table travels by value with the launch.
+struct TiledTube
+{
+    int64_t n;
+    int64_t seg;
+    int64_t steps;
+    int64_t blk0;
+    int last;
+    double hdt;
+    double qdt;
+    double const * coord;
+    double const * in0;
+    double const * in1;
+    double * out0;
+    double * out1;
+    double const * gamma;
+    double * cfl;
+};
+constexpr int MAX_TUBES = 16;
+struct TubeSet
+{
+    int nt;
+    TiledTube t[MAX_TUBES];
+};
+
+template <int ALPHA>
+__global__ void __launch_bounds__(TBLOCK) k_march_tiled_multi(TubeSet const ts)
+{
+    __shared__ double s0[TROWS * 3];
+    __shared__ double s1[TROWS * 3];
+    int k = 0;
+    for (int j = 1; j < ts.nt; ++j)
+    {
+        if (static_cast<int64_t>(blockIdx.x) >= ts.t[j].blk0)
+        {
+            k = j;
+        }
+    }
+    TiledTube const & tb = ts.t[k];
+    tiled_segment<ALPHA>(static_cast<int64_t>(blockIdx.x) - tb.blk0, s0, s1, tb.n, tb.seg, 2 * tb.steps + 2,
+                         tb.steps, tb.last, tb.hdt, tb.qdt, tb.coord, tb.in0, tb.in1, tb.out0, tb.out1, tb.gamma,
+                         tb.cfl);
+}
+
 // Derived fields (Euler1DCore.hpp:106-168, Euler1DCore.cpp:37-68,174-205).
 __global__ void k_derived(int which, int64_t n, double const * so0, double const * gamma, double * out)
 {
@@ -503,6 +563,47 @@ __global__ void k_derived(int which, int64_t n, double const * so0, double const
 }
 
 } /* end namespace */
+
+// The tiled march: segments of 64-512 rows (about n / 128), up to 64 steps
+// per launch with a halo of 2 steps + 2 rows.  The state moves between
+// (so0, so1) and the other pair (b0, b1) from launch to launch and ends in
+// (so0, so1); both pairs hold n * 3 values.  Enqueued on `stream`.
+void tiled_march(int64_t n, double hdt, double qdt, int alpha, int64_t steps, double const * coord,
+                 double const * gamma, double * cfl, double * so0, double * so1, double * b0, double * b1,
+                 hipStream_t stream)
+{
+    int64_t seg = (n + 127) / 128;
+    seg = seg < 64 ? 64 : seg > 512 ? 512 : seg;
+    int64_t const kmax = (TROWS - seg - 4) / 4; // seg + 2 (2 k + 2) <= TROWS
+    int64_t const kstep = kmax < 64 ? kmax : 64;
+    unsigned const nb = static_cast<unsigned>((n + seg - 1) / seg);
+    double * a0 = so0;
+    double * a1 = so1;
+    for (int64_t done = 0; done < steps;)
+    {
+        int64_t const k = steps - done < kstep ? steps - done : kstep;
+        int const last = done + k == steps ? 1 : 0;
+        if (alpha == 1)
+        {
+            k_march_tiled<1><<<nb, TBLOCK, 0, stream>>>(n, seg, 2 * k + 2, k, last, hdt, qdt, coord, a0, a1, b0,
+                                                        b1, gamma, cfl);
+        }
+        else
+        {
+            k_march_tiled<2><<<nb, TBLOCK, 0, stream>>>(n, seg, 2 * k + 2, k, last, hdt, qdt, coord, a0, a1, b0,
+                                                        b1, gamma, cfl);
+        }
+        SC_HIP(hipGetLastError());
+        std::swap(a0, b0);
+        std::swap(a1, b1);
+        done += k;
+    }
+    if (a0 != so0)
+    {
+        SC_HIP(hipMemcpyAsync(so0, a0, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, stream));
+        SC_HIP(hipMemcpyAsync(so1, a1, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, stream));
+    }
+}
 
 struct Core1D
 {
@@ -625,50 +726,18 @@ struct Core1D
         SC_HIP(hipGetLastError());
     }
 
-    // The tiled march: segments of 64-512 rows (about n / 128), up to 64
-    // steps per launch with a halo of 2 steps + 2 rows, the state moving
-    // between (so0, so1) and (tso0, tso1) from launch to launch.
+    // The tiled march (tiled_march below), the other buffer pair being
+    // (tso0, tso1), allocated on first use.
     void march_tiled(int alpha, int64_t steps)
     {
-        int64_t seg = (n + 127) / 128;
-        seg = seg < 64 ? 64 : seg > 512 ? 512 : seg;
-        int64_t const kmax = (TROWS - seg - 4) / 4; // seg + 2 (2 k + 2) <= TROWS
-        int64_t const kstep = kmax < 64 ? kmax : 64;
-        unsigned const nb = static_cast<unsigned>((n + seg - 1) / seg);
         if (tso0.ptr == nullptr)
         {
             AllocScope const scope(stream);
             tso0.alloc(n * 3);
             tso1.alloc(n * 3);
         }
-        double * a0 = so0.ptr;
-        double * a1 = so1.ptr;
-        double * b0 = tso0.ptr;
-        double * b1 = tso1.ptr;
-        for (int64_t done = 0; done < steps;)
-        {
-            int64_t const k = steps - done < kstep ? steps - done : kstep;
-            int const last = done + k == steps ? 1 : 0;
-            if (alpha == 1)
-            {
-                k_march_tiled<1><<<nb, TBLOCK, 0, stream>>>(n, seg, 2 * k + 2, k, last, hdt(), qdt(), coord.ptr, a0,
-                                                            a1, b0, b1, gamma.ptr, cfl.ptr);
-            }
-            else
-            {
-                k_march_tiled<2><<<nb, TBLOCK, 0, stream>>>(n, seg, 2 * k + 2, k, last, hdt(), qdt(), coord.ptr, a0,
-                                                            a1, b0, b1, gamma.ptr, cfl.ptr);
-            }
-            SC_HIP(hipGetLastError());
-            std::swap(a0, b0);
-            std::swap(a1, b1);
-            done += k;
-        }
-        if (a0 != so0.ptr)
-        {
-            SC_HIP(hipMemcpyAsync(so0.ptr, a0, so0.bytes(), hipMemcpyDeviceToDevice, stream));
-            SC_HIP(hipMemcpyAsync(so1.ptr, a1, so1.bytes(), hipMemcpyDeviceToDevice, stream));
-        }
+        tiled_march(n, hdt(), qdt(), alpha, steps, coord.ptr, gamma.ptr, cfl.ptr, so0.ptr, so1.ptr, tso0.ptr,
+                    tso1.ptr, stream);
     }
 
     int tiled = 1; // march with march_tiled (else the single-workgroup or per-half-step kernels)
@@ -735,9 +804,296 @@ void Core1D::fill_gamma(double g)
     SC_HIP(hipGetLastError());
 }
 
+// The request loop's 1D engine: admitted tubes (each a Core1D whose
+// tables it takes over) march together, one launch per round for all of
+// them (MAX_TUBES per launch), each tube advancing up to its own tiled
+// step count per round.  A tube that has run its steps has its so0, so1
+// and cfl drained to pinned host memory on the engine's stream.
+struct DServe1D
+{
+    StreamHolder sh; // declared first: destroyed last
+    hipStream_t stream = sh.s;
+
+    struct Slot
+    {
+        bool used = false;
+        bool marching = false;
+        Core1D * core = nullptr;
+        int alpha = 2;
+        int64_t steps_left = 0;
+        bool in_other = false; // state in (tso0, tso1)
+        double * host = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Slot> slots;
+    std::deque<hipEvent_t> chunks;
+
+    explicit DServe1D(int64_t nslot)
+    {
+        if (nslot < 1 || nslot > (1 << 20))
+        {
+            fail(SC_EVALUE, "1D serve engine: between 1 and 1048576 slots");
+        }
+        slots.resize(static_cast<size_t>(nslot));
+    }
+
+    ~DServe1D()
+    {
+        (void)hipStreamSynchronize(stream);
+        for (auto ev : chunks)
+        {
+            (void)hipEventDestroy(ev);
+        }
+        for (auto & sl : slots)
+        {
+            release(sl);
+        }
+    }
+
+    static void release(Slot & sl)
+    {
+        if (sl.done != nullptr)
+        {
+            (void)hipEventDestroy(sl.done);
+        }
+        if (sl.host != nullptr)
+        {
+            (void)hipHostFree(sl.host);
+        }
+        delete sl.core;
+        sl = Slot{};
+    }
+
+    // Take over a tube; returns the slot or -1 when every slot is in use.
+    int64_t admit(Core1D * c, int alpha, int64_t steps)
+    {
+        if (alpha != 1 && alpha != 2)
+        {
+            fail(SC_EVALUE, "alpha must be 1 or 2");
+        }
+        if (steps < 1)
+        {
+            fail(SC_EVALUE, "1D serve engine: a solve needs at least one step");
+        }
+        int64_t slot = -1;
+        for (size_t k = 0; k < slots.size(); ++k)
+        {
+            if (!slots[k].used)
+            {
+                slot = static_cast<int64_t>(k);
+                break;
+            }
+        }
+        if (slot < 0)
+        {
+            return -1;
+        }
+        if (c->tso0.ptr == nullptr)
+        {
+            AllocScope const scope(c->stream);
+            c->tso0.alloc(c->n * 3);
+            c->tso1.alloc(c->n * 3);
+        }
+        Slot sl;
+        SC_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.host), sizeof(double) * c->n * 7));
+        hipEvent_t ev = nullptr;
+        try
+        {
+            SC_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+            SC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            SC_HIP(hipEventRecord(ev, c->stream));
+            SC_HIP(hipStreamWaitEvent(stream, ev, 0));
+            (void)hipEventDestroy(ev);
+        }
+        catch (...)
+        {
+            if (ev != nullptr)
+            {
+                (void)hipEventDestroy(ev);
+            }
+            release(sl);
+            throw;
+        }
+        sl.used = true;
+        sl.marching = true;
+        sl.core = c;
+        sl.alpha = alpha;
+        sl.steps_left = steps;
+        slots[slot] = sl;
+        return slot;
+    }
+
+    void drain(Slot & sl)
+    {
+        Core1D & c = *sl.core;
+        double const * a0 = sl.in_other ? c.tso0.ptr : c.so0.ptr;
+        double const * a1 = sl.in_other ? c.tso1.ptr : c.so1.ptr;
+        SC_HIP(hipMemcpyAsync(sl.host, a0, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, stream));
+        SC_HIP(hipMemcpyAsync(sl.host + 3 * c.n, a1, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, stream));
+        SC_HIP(hipMemcpyAsync(sl.host + 6 * c.n, c.cfl.ptr, sizeof(double) * c.n, hipMemcpyDeviceToHost, stream));
+        SC_HIP(hipEventRecord(sl.done, stream));
+    }
+
+    // One round: every marching tube advances min(its tiled step count,
+    // the steps it has left).
+    void step()
+    {
+        for (int alpha = 1; alpha <= 2; ++alpha)
+        {
+            TubeSet ts{};
+            std::vector<int64_t> members;
+            int64_t blocks = 0;
+            auto flush = [&]()
+            {
+                if (ts.nt == 0)
+                {
+                    return;
+                }
+                unsigned const nb = static_cast<unsigned>(blocks);
+                if (alpha == 1)
+                {
+                    k_march_tiled_multi<1><<<nb, TBLOCK, 0, stream>>>(ts);
+                }
+                else
+                {
+                    k_march_tiled_multi<2><<<nb, TBLOCK, 0, stream>>>(ts);
+                }
+                SC_HIP(hipGetLastError());
+                for (int64_t k : members)
+                {
+                    Slot & sl = slots[k];
+                    sl.in_other = !sl.in_other;
+                    if (sl.steps_left == 0)
+                    {
+                        sl.marching = false;
+                        drain(sl);
+                    }
+                }
+                ts = TubeSet{};
+                members.clear();
+                blocks = 0;
+            };
+            for (size_t k = 0; k < slots.size(); ++k)
+            {
+                Slot & sl = slots[k];
+                if (!sl.marching || sl.alpha != alpha)
+                {
+                    continue;
+                }
+                Core1D & c = *sl.core;
+                int64_t seg = (c.n + 127) / 128;
+                seg = seg < 64 ? 64 : seg > 512 ? 512 : seg;
+                int64_t const kmax = (TROWS - seg - 4) / 4;
+                int64_t const kstep = kmax < 64 ? kmax : 64;
+                int64_t const kk = sl.steps_left < kstep ? sl.steps_left : kstep;
+                TiledTube & t = ts.t[ts.nt++];
+                t.n = c.n;
+                t.seg = seg;
+                t.steps = kk;
+                t.blk0 = blocks;
+                t.last = kk == sl.steps_left ? 1 : 0;
+                t.hdt = c.hdt();
+                t.qdt = c.qdt();
+                t.coord = c.coord.ptr;
+                t.in0 = sl.in_other ? c.tso0.ptr : c.so0.ptr;
+                t.in1 = sl.in_other ? c.tso1.ptr : c.so1.ptr;
+                t.out0 = sl.in_other ? c.so0.ptr : c.tso0.ptr;
+                t.out1 = sl.in_other ? c.so1.ptr : c.tso1.ptr;
+                t.gamma = c.gamma.ptr;
+                t.cfl = c.cfl.ptr;
+                blocks += (c.n + seg - 1) / seg;
+                sl.steps_left -= kk;
+                members.push_back(static_cast<int64_t>(k));
+                if (ts.nt == MAX_TUBES)
+                {
+                    flush();
+                }
+            }
+            flush();
+        }
+        hipEvent_t ev = nullptr;
+        SC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        SC_HIP(hipEventRecord(ev, stream));
+        chunks.push_back(ev);
+    }
+
+    int64_t pending()
+    {
+        while (!chunks.empty())
+        {
+            hipError_t const e = hipEventQuery(chunks.front());
+            if (e == hipErrorNotReady)
+            {
+                break;
+            }
+            SC_HIP(e);
+            (void)hipEventDestroy(chunks.front());
+            chunks.pop_front();
+        }
+        return static_cast<int64_t>(chunks.size());
+    }
+
+    Slot & at(int64_t slot)
+    {
+        if (slot < 0 || slot >= static_cast<int64_t>(slots.size()))
+        {
+            fail(SC_EINDEX, "1D serve engine: slot out of range");
+        }
+        Slot & sl = slots[slot];
+        if (!sl.used)
+        {
+            fail(SC_EVALUE, "1D serve engine: slot " + std::to_string(slot) + " holds no solve");
+        }
+        return sl;
+    }
+
+    int query(int64_t slot)
+    {
+        Slot & sl = at(slot);
+        if (sl.marching)
+        {
+            return 0;
+        }
+        hipError_t const e = hipEventQuery(sl.done);
+        if (e == hipErrorNotReady)
+        {
+            return 0;
+        }
+        SC_HIP(e);
+        return 1;
+    }
+
+    // so0 [n*3], so1 [n*3], cfl [n] of a finished tube (any may be null);
+    // frees the slot and the tube.
+    void result(int64_t slot, double * so0, double * so1, double * cfl)
+    {
+        Slot & sl = at(slot);
+        if (sl.marching)
+        {
+            fail(SC_ERUNTIME, "1D serve engine: slot " + std::to_string(slot) + " is still marching");
+        }
+        SC_HIP(hipEventSynchronize(sl.done));
+        int64_t const n = sl.core->n;
+        if (so0 != nullptr)
+        {
+            std::memcpy(so0, sl.host, sizeof(double) * 3 * n);
+        }
+        if (so1 != nullptr)
+        {
+            std::memcpy(so1, sl.host + 3 * n, sizeof(double) * 3 * n);
+        }
+        if (cfl != nullptr)
+        {
+            std::memcpy(cfl, sl.host + 6 * n, sizeof(double) * n);
+        }
+        release(sl);
+    }
+};
+
 } /* end namespace scamd */
 
 using scamd::Core1D;
+using scamd::DServe1D;
 using scamd::guard;
 
 extern "C"
@@ -962,6 +1318,39 @@ int sc1d_derived(void * h, int which, double * host_out)
             SC_HIP(hipMemcpyAsync(host_out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, a.stream));
             a.sync();
         });
+}
+
+int scserve1d_create(int64_t max_slots, void ** out)
+{
+    return guard([&]() { *out = new DServe1D(max_slots); });
+}
+
+void scserve1d_destroy(void * a) { delete static_cast<DServe1D *>(a); }
+
+int scserve1d_admit(void * a, void * core, int alpha, int64_t steps, int64_t * slot)
+{
+    return guard(
+        [&]() { *slot = static_cast<DServe1D *>(a)->admit(static_cast<Core1D *>(core), alpha, steps); });
+}
+
+int scserve1d_step(void * a)
+{
+    return guard([&]() { static_cast<DServe1D *>(a)->step(); });
+}
+
+int scserve1d_pending(void * a, int64_t * chunks)
+{
+    return guard([&]() { *chunks = static_cast<DServe1D *>(a)->pending(); });
+}
+
+int scserve1d_query(void * a, int64_t slot, int * done)
+{
+    return guard([&]() { *done = static_cast<DServe1D *>(a)->query(slot); });
+}
+
+int scserve1d_result(void * a, int64_t slot, double * so0, double * so1, double * cfl)
+{
+    return guard([&]() { static_cast<DServe1D *>(a)->result(slot, so0, so1, cfl); });
 }
 
 } /* end extern "C" */

@@ -16,7 +16,8 @@ def _declared():
 
 def test_header_declares_entry_points():
     names = _declared()
-    for pre in ('sc_', 'sc1d_', 'scmesh_', 'sceuler_', 'scge_', 'scst_'):
+    for pre in ('sc_', 'sc1d_', 'scmesh_', 'sceuler_', 'scge_', 'scst_',
+                'scserve_', 'scserve1d_'):
         assert any(n.startswith(pre) for n in names), pre
 
 
