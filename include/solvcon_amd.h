@@ -86,6 +86,19 @@ int sc1d_query(void * h, int * done);
  * (which = 0..5) into host_out[ncoord], Euler1DCore.hpp:106-168. */
 int sc1d_derived(void * h, int which, double * host_out);
 
+/* No counterpart in the reference.  march_alpha<alpha>(steps) of a tube
+ * whose tables the caller holds in device memory (coord, gamma, cfl [n],
+ * so0, so1 [n*3], row-major as above), enqueued on `stream` (a hipStream_t;
+ * NULL for the default stream) without allocating anything: for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving
+ * loops).  The other buffer pair of the tiled march is a file-scope device
+ * array of 2 x 3 x 131072 doubles (6 MB, held by every GPU the library runs
+ * on until the process ends), so ncoord is at most 131071; larger tubes are
+ * rejected with SC_EVALUE.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU race.  Results equal sc1d_march's bit for bit. */
+int sc1d_march_noalloc(double const * coord, double const * gamma, double * so0, double * so1, double * cfl,
+                       int64_t ncoord, double time_increment, int alpha, int64_t steps, void * stream);
+
 /* -------------------------------------------------- mesh (StaticMesh) ---- */
 /* Table ids for scmesh_*: 0 ndcrd, 1 fccnd, 2 fcnml, 3 fcara, 4 clcnd,
  * 5 clvol, 6 fctpn, 7 cltpn, 8 clgrp, 9 fcnds, 10 fccls, 11 clnds,

@@ -25,6 +25,7 @@
 #include <memory>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <vector>
 #include <utility>
 
@@ -804,6 +805,37 @@ void Core1D::fill_gamma(double g)
     SC_HIP(hipGetLastError());
 }
 
+// The march that allocates nothing (sc1d_march_noalloc): the other buffer
+// pair of the tiled march lives in this file-scope device array, one copy
+// per GPU, for tubes of at most NOALLOC_MAX points.  Host code never uses
+// the array's name as an address: the address of each device's copy comes
+// from hipGetSymbolAddress, once per device.
+constexpr int64_t NOALLOC_MAX = 131071;
+__device__ __align__(256) double scratch[2 * 3 * (NOALLOC_MAX + 1)];
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex noalloc_mu;
+double * noalloc_ptr[NOALLOC_DEVICES] = {};
+
+// The current device's copy of the scratch array.
+double * noalloc_scratch()
+{
+    int dev = 0;
+    SC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_DEVICES)
+    {
+        fail(SC_ERUNTIME, "sc1d_march_noalloc: device index out of range");
+    }
+    std::lock_guard<std::mutex> const lock(noalloc_mu);
+    if (noalloc_ptr[dev] == nullptr)
+    {
+        void * ptr = nullptr;
+        SC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        noalloc_ptr[dev] = static_cast<double *>(ptr);
+    }
+    return noalloc_ptr[dev];
+}
+
 // The request loop's 1D engine: admitted tubes (each a Core1D whose
 // tables it takes over) march together, one launch per round for all of
 // them (MAX_TUBES per launch), each tube advancing up to its own tiled
@@ -1317,6 +1349,46 @@ int sc1d_derived(void * h, int which, double * host_out)
             a.derived(which, tmp.ptr);
             SC_HIP(hipMemcpyAsync(host_out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, a.stream));
             a.sync();
+        });
+}
+
+int sc1d_march_noalloc(double const * coord, double const * gamma, double * so0, double * so1, double * cfl,
+                       int64_t ncoord, double time_increment, int alpha, int64_t steps, void * stream)
+{
+    return guard(
+        [&]()
+        {
+            if (ncoord % 2 == 0)
+            {
+                scamd::fail(scamd::SC_EVALUE, "ncoord cannot be even");
+            }
+            if (ncoord < 5)
+            {
+                scamd::fail(scamd::SC_EVALUE, "ncoord must be at least 5");
+            }
+            if (ncoord > scamd::NOALLOC_MAX)
+            {
+                scamd::fail(scamd::SC_EVALUE, "sc1d_march_noalloc: ncoord " + std::to_string(ncoord) +
+                                                  " exceeds the scratch maximum of " +
+                                                  std::to_string(scamd::NOALLOC_MAX) + " points");
+            }
+            if (alpha != 1 && alpha != 2)
+            {
+                scamd::fail(scamd::SC_EVALUE, "alpha must be 1 or 2");
+            }
+            if (steps < 0)
+            {
+                scamd::fail(scamd::SC_EVALUE, "steps must be >= 0");
+            }
+            if (coord == nullptr || gamma == nullptr || so0 == nullptr || so1 == nullptr || cfl == nullptr)
+            {
+                scamd::fail(scamd::SC_EVALUE, "sc1d_march_noalloc: null array");
+            }
+            double * const b0 = scamd::noalloc_scratch();
+            double * const b1 = b0 + 3 * (scamd::NOALLOC_MAX + 1);
+            double const hdt = time_increment / 2.0;
+            scamd::tiled_march(ncoord, hdt, hdt / 2.0, alpha, steps, coord, gamma, cfl, so0, so1, b0, b1,
+                               static_cast<hipStream_t>(stream));
         });
 }
 
