@@ -61,7 +61,31 @@ def _alone(defn, state, dt, steps, gamma=1.4):
     return mh, ec
 
 
-def _check_batch(defs, states, steps, chunk=0):
+def _oracle_alone(orc, defn, state, dt, steps, gamma=1.4):
+    """The oracle's march of one instance: its own mesh, state, dt and
+    step count, non-reflective boundaries."""
+    o = orc.make_mesh(*defn)
+    oe = orc.Euler(o, float(dt))
+    oe.init_solution(gamma, 1.0, [0.0] * o.ndim, 1.0)
+    oe.so0n[o.ngstcell:] = state
+    oe.add_bc(orc.BC_NONREFL, o.bndfcs[:, 0].copy())
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    oe.march(int(steps))
+    return o, oe
+
+
+def _check_oracle(orc, b, k, defn, state, steps, gamma=1.4):
+    # Every body row of the batched instance equals the oracle's alone.
+    _, oe = _oracle_alone(orc, defn, state, b.dts[k], steps, gamma)
+    for name in ('so0n', 'so1n', 'cflo', 'cflc'):
+        want = getattr(oe, name)[oe.ngstcell:]
+        got = b.body(name, k)
+        assert np.isfinite(want).all(), (k, name)
+        assert np.array_equal(got, want), (k, name, 'oracle')
+
+
+def _check_batch(defs, states, steps, chunk=0, orc=None):
     from solvcon_amd.batch import BatchedEuler
     b = BatchedEuler(defs, states, steps, chunk_cells=chunk)
     assert b.core.batch_chunk == chunk
@@ -70,6 +94,8 @@ def _check_batch(defs, states, steps, chunk=0):
     for k, d in enumerate(defs):
         print('  instance', k, 'cells', len(d[2]), 'steps', steps[k],
               flush=True)
+        if orc is not None:
+            _check_oracle(orc, b, k, d, states[k], steps[k])
         mh, ec = _alone(d, states[k], b.dts[k], steps[k])
         ng = mh.ngstcell
         for name in TABLES:
