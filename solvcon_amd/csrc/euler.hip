@@ -423,20 +423,42 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     // its row, where every staged lane reads the block's first row instead;
     // the lanes of one load then touch only the cache lines of the
     // neighbours that are not staged.
-    double jce[NF][ND];
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
+    //
+    // In 3D the solution points and, when so0t is read from memory, the
+    // neighbours' so0t are loaded again where they are used instead of
+    // being held in registers from the flux loop to the gradient: the same
+    // values (the tables are not written by this kernel), a shorter live
+    // range, fewer spills.
+    constexpr bool RELOAD_CEN = ND == 3;
+    constexpr bool RELOAD_ST = ND == 3 && ((MODE & M_STMEM) != 0 || (MODE & M_SOLN) == 0);
+    auto cen_at = [&](int f, int d) -> double
     {
         bool const inb = lr[f] >= 0;
         double const * cg = p.cen + (inb ? rbase : nb[f]) * ND;
         double const * cl = lds + O_CN + (inb ? lr[f] : 0) * ND;
+        return pick(inb, cl[d], cg[d]);
+    };
+    auto st_at = [&](int f, int k) -> double
+    {
+        bool const inb = lr[f] >= 0;
+        int const l = inb ? lr[f] : 0;
+        int64_t const jg = inb ? rbase : nb[f];
+        return pick(inb, lds[O_ST + l * NEQ + k], p.st[jg * NEQ + k]);
+    };
+    double jce[RELOAD_CEN ? 1 : NF][ND];
+    if constexpr (!RELOAD_CEN)
+    {
 #pragma unroll
-        for (int d = 0; d < ND; ++d)
+        for (int f = 0; f < NF; ++f)
         {
-            jce[f][d] = pick(inb, cl[d], cg[d]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                jce[f][d] = cen_at(f, d);
+            }
         }
     }
-    double st[NF][NEQ];
+    double st[RELOAD_ST ? 1 : NF][NEQ];
     double u0[NEQ];
 
     if constexpr ((MODE & M_SOLN) != 0)
@@ -472,12 +494,28 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                     j1[k][d] = pick(inb, lds[O_S1 + l * (NEQ * ND) + k * ND + d], p.s1c[jg * (NEQ * ND) + k * ND + d]);
                 }
             }
+            // This face's neighbour so0t (from memory or recomputed) and
+            // solution point.
+            double stf[NEQ];
             if constexpr ((MODE & M_STMEM) != 0)
             {
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    st[f][k] = pick(inb, lds[O_ST + l * NEQ + k], p.st[jg * NEQ + k]);
+                    stf[k] = pick(inb, lds[O_ST + l * NEQ + k], p.st[jg * NEQ + k]);
+                }
+            }
+            double jcef[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+            {
+                if constexpr (RELOAD_CEN)
+                {
+                    jcef[d] = cen_at(f, d);
+                }
+                else
+                {
+                    jcef[d] = jce[f][d];
                 }
             }
             double const jgam = pick(inb, lds[O_GM + l], p.gam[jg]);
@@ -489,7 +527,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
-                    fusp += (geo[(LY::FB + f * ND + d) * n] - jce[f][d]) * j1[k][d];
+                    fusp += (geo[(LY::FB + f * ND + d) * n] - jcef[d]) * j1[k][d];
                 }
                 acc[k] += fusp * bvol;
             }
@@ -497,7 +535,15 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             jac.set(jgam, js);
             if constexpr ((MODE & M_STMEM) == 0)
             {
-                solt_of<ND>(jac, j1, st[f]);
+                solt_of<ND>(jac, j1, stf);
+            }
+            if constexpr (!RELOAD_ST)
+            {
+#pragma unroll
+                for (int k = 0; k < NEQ; ++k)
+                {
+                    st[f][k] = stf[k];
+                }
             }
             constexpr int NS = TR::nsub(f);
             // The solution at every sub-face centroid of this face, then
@@ -512,11 +558,11 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    usfc[s][k] = qdt * st[f][k];
+                    usfc[s][k] = qdt * stf[k];
 #pragma unroll
                     for (int d = 0; d < ND; ++d)
                     {
-                        usfc[s][k] += (gsub(LY::FC + sf * ND + d) - jce[f][d]) * j1[k][d];
+                        usfc[s][k] += (gsub(LY::FC + sf * ND + d) - jcef[d]) * j1[k][d];
                     }
                 }
             }
@@ -679,18 +725,15 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 u0[k] = p.s0n[(row) * NEQ + k];
             }
         }
-        if constexpr ((MODE & M_SOLN) == 0)
+        if constexpr ((MODE & M_SOLN) == 0 && !RELOAD_ST)
         {
 #pragma unroll
             for (int f = 0; f < NF; ++f)
             {
-                bool const inb = lr[f] >= 0;
-                int const l = inb ? lr[f] : 0;
-                int64_t const jg = inb ? rbase : nb[f];
 #pragma unroll
                 for (int k = 0; k < NEQ; ++k)
                 {
-                    st[f][k] = pick(inb, lds[O_ST + l * NEQ + k], p.st[jg * NEQ + k]);
+                    st[f][k] = st_at(f, k);
                 }
             }
         }
@@ -711,7 +754,16 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             for (int d = 0; d < ND; ++d)
             {
                 double const mid = geo[(LY::FB + f * ND + d) * n];
-                gp[f][d] = mid + tau * (jce[f][d] - mid);
+                double jc;
+                if constexpr (RELOAD_CEN)
+                {
+                    jc = cen_at(f, d);
+                }
+                else
+                {
+                    jc = jce[f][d];
+                }
+                gp[f][d] = mid + tau * (jc - mid);
             }
         }
         double avg[ND];
@@ -791,7 +843,16 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             for (int d = 0; d < ND; ++d)
             {
                 idis[f][d] = gp[f][d] - cnd[d];
-                jdis[d] = (gp[f][d] - jce[f][d]) + ice[d] - cnd[d];
+                double jc;
+                if constexpr (RELOAD_CEN)
+                {
+                    jc = cen_at(f, d);
+                }
+                else
+                {
+                    jc = jce[f][d];
+                }
+                jdis[d] = (gp[f][d] - jc) + ice[d] - cnd[d];
             }
             int64_t const j = nb[f];
             bool const inb = lr[f] >= 0;
@@ -800,7 +861,16 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                double val = pick(inb, lds[l * NEQ + k], p.s0c[jg * NEQ + k]) + hdt * st[f][k] - u0[k];
+                double stv;
+                if constexpr (RELOAD_ST)
+                {
+                    stv = st_at(f, k);
+                }
+                else
+                {
+                    stv = st[f][k];
+                }
+                double val = pick(inb, lds[l * NEQ + k], p.s0c[jg * NEQ + k]) + hdt * stv - u0[k];
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
