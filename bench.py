@@ -1,30 +1,32 @@
 """Benchmark of the CESE Euler march on one MI355X.
 
-Default workload (``--config 2d``): the 2D Euler CESE march on the jittered
-triangle mesh of the unit square.
-- Mesh: 1024 x 1024 quads split into 2,097,152 triangles; interior nodes
-  jittered by up to 0.2h; seed 2.
-- Initial state: a four-quadrant Riemann box with rho and p drawn from
-  U(0.1, 1.5), u = v = 0, gamma = 1.4.
-- Non-reflective boundaries; dt set so that the largest initial CFL number
-  is 0.8.
-A step is one CESE step (two substeps) over the whole mesh. ``value`` is
-cell-updates/s with the state resident in HBM.
+Default workload (``--config 3d``): the 3D Euler CESE march on the mixed
+hex/tet/prism/pyramid block of the unit cube.
+- Mesh: 200^3 hexes, 30% in tetrahedral layers and 20% in prismatic ones,
+  with a pyramid transition layer: 19,280,000 cells.
+- Initial state: a spherical blast (rho = 1, p = 10 inside r = 0.1 of the
+  centre, p = 0.1 outside), u = 0, gamma = 1.4.
+- Slip walls on x = 0, y = 0 and z = 0, non-reflective boundaries
+  elsewhere; dt set so that the largest initial CFL number is 0.8.
+A step is one CESE step (two substeps) over the whole mesh.  ``value`` is
+cell-updates/s with the state resident in HBM.  The same JSON line carries
+the batched form (``batched``: 4096 independent 2D solves in one batched
+march against the same solves marched one by one with plain solvers) unless
+``--no-batched``.
 
 Other modes:
-- ``--config 3d``: the mixed hex/tet/prism/pyramid block of the unit cube
-  (200^3 hexes, 30% tetrahedral and 20% prismatic layers). It carries a
-  spherical blast (rho = 1, p = 10 inside r = 0.1, p = 0.1 outside), slip
-  walls on x = 0, y = 0 and z = 0, and non-reflective boundaries elsewhere.
-  It runs 200 steps.
+- ``--config 2d``: the 2D march on the jittered triangle mesh of the unit
+  square (1024 x 1024 quads split into 2,097,152 triangles, Riemann box,
+  non-reflective boundaries, 500 steps).
 - ``--config 1d``: the Sod tube, 2001 points, 600 steps of march_alpha2.
-- ``--batched``: thousands of independent 2D solves in one batched march,
-  against the plain form.
+- ``--batched``: the batched form alone, as its own JSON line.
 - ``--served``: a Poisson request stream of 1D/2D/3D solves through the
-  request loop, with p50/p99 latency, against serial plain execution.
+  request loop (continuous batching), with p50/p99 latency, against serial
+  plain execution.
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K]
-[--warmup W] [--config 2d|3d|1d] [--batched] [--served] [--no-cpu]
+[--warmup W] [--config 3d|2d|1d] [--batched] [--served] [--no-cpu]
+[--no-batched]
 """
 
 import argparse
@@ -141,13 +143,38 @@ class _View(object):
         self.ndarray = a
 
 
-def cpu_baseline_mesh(defn, state, bcs_of, dt, steps, threads):
+def host_threads():
+    """Host threads this process may use: OMP_NUM_THREADS when set (the
+    GPU box sets it to the CPU share of its one GPU), else the CPUs of the
+    process's affinity mask."""
+    env = int(os.environ.get('OMP_NUM_THREADS', '0') or 0)
+    if env > 0:
+        return env
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline_mesh(defn, state, bcs_of, dt, plan):
     """The oracle (the CPU restatement of the reference march) on the given
-    mesh, state and boundaries, timed over a few steps."""
+    mesh, state and boundaries: for each (threads, steps) of ``plan`` one
+    untimed step, then the steps timed.  Returns [(cell-updates/s,
+    seconds)], and the cell count."""
     import numpy as np
     from oracle import oracle as O
     m = O.make_mesh(*defn)
-    oe = O.Euler(m, dt, nthreads=threads)
+    oe = O.Euler(m, dt, nthreads=plan[0][0])
     oe.init_solution(1.4, 1.0, [0.0] * m.ndim, 1.0)
     ngst = oe.ngstcell
     oe.so0n[ngst:] = state(m.clcnd[ngst:])
@@ -160,11 +187,15 @@ def cpu_baseline_mesh(defn, state, bcs_of, dt, steps, threads):
         oe.add_bc(kinds[kind], np.asarray(faces))
     oe.call('bc_soln')
     oe.call('bc_dsoln')
-    oe.march(1)
-    t0 = time.perf_counter()
-    oe.march(steps)
-    el = time.perf_counter() - t0
-    return oe.ncell * steps / el, el
+    out = []
+    for threads, steps in plan:
+        oe.set_threads(threads)
+        oe.march(1)
+        t0 = time.perf_counter()
+        oe.march(steps)
+        el = time.perf_counter() - t0
+        out.append((oe.ncell * steps / el, el))
+    return out, oe.ncell
 
 
 def group_mix(ec):
@@ -224,23 +255,36 @@ def run_mesh(args, which):
             'traffic': traffic_from_profiles(traffic_name)}
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get('OMP_NUM_THREADS', '0') or 0) or \
-            min(os.cpu_count() or 1, 16)
+        nt = host_threads()
         if which == '2d':
-            cdefn, csteps = defn, args.cpu_steps
-            sample = '%d steps of the same %d-cell mesh and state'
+            cdefn = defn
+            plan = [(nt, args.cpu_steps), (1, max(1, args.cpu_steps // 12))]
+            what = 'the same %d-cell mesh and state'
         else:
             # The full 3D mesh takes minutes to build on the CPU: the
-            # sample is the same generator and state at a smaller size.
+            # sample is the same generator, state and boundaries at a
+            # smaller size.
             cdefn = workload_3d(args.cpu_size3)[0]
-            csteps = args.cpu_steps3
-            sample = '%d steps of the same generator and state at %d cells'
-        log('bench: timing the oracle on %d threads' % threads)
-        cval, cel = cpu_baseline_mesh(cdefn, state, bcs, dt, csteps, threads)
-        cpu = {'value': round(cval, 1), 'unit': 'cell-updates/s',
-               'cores': threads, 'kind': 'port',
-               'sample': (sample % (csteps, len(cdefn[2])))
-               + ' (%.1f s), OpenMP over cells' % cel}
+            plan = [(nt, args.cpu_steps3), (1, args.cpu_steps3_1t)]
+            what = 'the same generator, state and boundaries at %d cells'
+        log('bench: timing the oracle on %d threads and on 1' % nt)
+        res, ccells = cpu_baseline_mesh(cdefn, state, bcs, dt, plan)
+        (vn, en), (v1, e1) = res
+        cpu = {'value': round(vn, 1), 'unit': 'cell-updates/s',
+               'cores': nt, 'kind': 'port',
+               'sample': ('%d steps of ' + what + ' (%.1f s), OpenMP over '
+                          'cells on %d threads') % (plan[0][1], ccells, en,
+                                                    nt),
+               'cpu_model': cpu_model(),
+               'single_thread': {
+                   'value': round(v1, 1), 'cores': 1,
+                   'sample': ('%d steps of ' + what + ' (%.1f s)')
+                   % (plan[1][1], ccells, e1)}}
+        log('bench: oracle %.4g cell-updates/s on %d threads, %.4g on 1 '
+            '(%s)' % (vn, nt, v1, cpu['cpu_model']))
+    batched = None
+    if which == '3d' and not args.no_batched:
+        batched = run_batched(args, line=False)
     out = {
         'metric': METRIC,
         'value': round(value, 1),
@@ -264,10 +308,14 @@ def run_mesh(args, which):
             'mass': mass,
             'stream_copy_gbs': None if copy_gbs is None else round(copy_gbs, 1),
             'cfl_range': [cmin, cmax],
+            'kernel_forms': {CELL_NAME[t]: f for t, f in
+                             (ec.kernel_forms or {}).items()},
         },
         'roofline': roof,
         'cpu_baseline': cpu,
     }
+    if batched is not None:
+        out['batched'] = batched
     print(json.dumps(out), flush=True)
 
 
@@ -317,13 +365,21 @@ def run_1d(args):
     print(json.dumps(out), flush=True)
 
 
-def run_batched(args):
+def run_batched(args, line=True):
     """The batched form: ``--batch-n`` independent 2D triangle solves.
     - Mesh sizes are log-uniform in [1e3, 2e5] cells.
     - Step counts are uniform in [100, 1000].
-    - Each instance has its own seeded Riemann box.
-    All instances run in one batched march. The plain form is timed on a
-    sample of the same instances solved one after another."""
+    - Each instance has its own seeded Riemann box and its own dt, from its
+      initial CFL number ``--batch-cfl``.
+    All instances run in one batched march.  The plain form is measured on
+    a stratified sample: the instances are ordered by work (cells x steps)
+    and cut into ``--plain-sample`` strata of equal size, and the middle
+    instance of each stratum is set up and marched alone by a plain
+    EulerCore (kernel form fixed, so no autotuning is timed).  The serial
+    plain time of the whole batch is the stratified estimate: each
+    stratum's size times its sample's time.  Instances whose state is not
+    finite at the end are counted and left out of solves/s.
+    Returns the result dict; with ``line`` also prints it as a JSON line."""
     import numpy as np
     from solvcon_amd import _lib
     from solvcon_amd.batch import (batch_config, config_defs, config_states,
@@ -334,7 +390,7 @@ def run_batched(args):
     defs = config_defs(m, mseed)
     states = config_states(rseed)
     t1 = time.perf_counter()
-    b = BatchedEuler(defs, states, steps)
+    b = BatchedEuler(defs, states, steps, cfl=args.batch_cfl)
     _lib.check(_lib.lib().sc_device_sync())
     t2 = time.perf_counter()
     log('bench: batch of %d instances, %d cells, %.3g cell-updates; '
@@ -345,51 +401,72 @@ def run_batched(args):
     t4 = time.perf_counter()
     done = b.done_ms()
     wall = t4 - t3
+    so0 = b.core.so0n.ndarray[b.ngst:]
+    fin = np.array([np.isfinite(so0[b.coff[k]:b.coff[k + 1]]).all()
+                    for k in range(b.ninst)])
+    del so0
+    nbad = int((~fin).sum())
     log('bench: batched march %.3f s wall (%.1f ms device), %.4g '
-        'cell-updates/s, %.1f solves/s' %
-        (wall, dev_ms, b.cell_updates / wall, b.ninst / wall))
-    # The plain form on a sample spread over the sizes: each instance gets
-    # its own mesh and EulerCore and marches alone.
-    order = np.argsort(b.ncells)
-    pick = order[np.linspace(0, b.ninst - 1, args.plain_sample).astype(int)]
-    tm = []
-    for k in pick:
-        defn = defs[k]
-        ta = time.perf_counter()
-        mh = build_mesh(defn)
-        ec = EulerCore(mh, float(b.dts[k]))
-        ec.init_solution(1.4, 1.0, [0.0, 0.0], 1.0)
-        cen = mh.clcnd.ndarray[mh.ngstcell:]
-        ec.so0n.ndarray[mh.ngstcell:] = states(cen, k)
-        ec.add_nonrefl(mh.bndfcs.ndarray[:, 0])
-        ec.bc_soln()
-        ec.bc_dsoln()
-        tb = time.perf_counter()
-        ec.march(int(steps[k]))
-        tc = time.perf_counter()
-        # NaN-aware: an instance that goes unstable does so identically.
-        same = np.array_equal(ec.so0n.ndarray[mh.ngstcell:],
-                              b.body('so0n', k), equal_nan=True)
-        tm.append((b.ncells[k], steps[k], tb - ta, tc - tb, same))
-    tm = np.array(tm, dtype=float)
-    # Serial plain time of the whole batch from a fit of the sample:
-    # march time = a * steps + c * cells * steps (launch cost plus work).
-    A = np.stack([tm[:, 1], tm[:, 0] * tm[:, 1]], axis=1)
-    coef, *_ = np.linalg.lstsq(A, tm[:, 3], rcond=None)
-    plain_march = float(coef[0] * steps.sum()
-                        + coef[1] * (b.ncells * steps).sum())
-    setup_fit = np.polyfit(tm[:, 0], tm[:, 2], 1)
-    plain_setup = float(np.polyval(setup_fit, b.ncells).sum())
-    log('bench: plain sample of %d: march fit %.3g s/step + %.3g s per '
-        'cell-update; whole batch serially %.1f s march + %.1f s set-up; '
-        'all equal: %s' % (len(pick), coef[0], coef[1], plain_march,
-                           plain_setup, bool(tm[:, 4].all())))
-    # Latency of each solve, all submitted at the march's start: batched,
-    # the device time until the instance's last step ran; serially, the
-    # fitted march times of the instances before it and its own, in
-    # instance order.
-    lat = done / 1e3
-    slat = np.cumsum(coef[0] * steps + coef[1] * b.ncells * steps)
+        'cell-updates/s, %.1f solves/s, %d of %d instances not finite' %
+        (wall, dev_ms, b.cell_updates / wall, fin.sum() / wall, nbad,
+         b.ninst))
+    # The plain form on a stratified sample.
+    work = b.ncells * steps
+    order = np.argsort(work, kind='stable')
+    strata = np.array_split(order, args.plain_sample)
+    prev_form = os.environ.get('SC_EULER_FORM')
+    os.environ['SC_EULER_FORM'] = args.plain_form
+    rows = []
+    try:
+        for st in strata:
+            k = int(st[len(st) // 2])
+            ta = time.perf_counter()
+            mh = build_mesh(defs[k])
+            ec = EulerCore(mh, float(b.dts[k]))
+            ec.init_solution(1.4, 1.0, [0.0, 0.0], 1.0)
+            cen = mh.clcnd.ndarray[mh.ngstcell:]
+            ec.so0n.ndarray[mh.ngstcell:] = states(cen, k)
+            ec.add_nonrefl(mh.bndfcs.ndarray[:, 0])
+            ec.bc_soln()
+            ec.bc_dsoln()
+            _lib.check(_lib.lib().sc_device_sync())
+            tb = time.perf_counter()
+            ec.march(int(steps[k]))
+            tc = time.perf_counter()
+            # Bit for bit; a non-finite instance carries its NaNs to the
+            # same places alone and in the batch.
+            same = np.array_equal(ec.so0n.ndarray[mh.ngstcell:],
+                                  b.body('so0n', k),
+                                  equal_nan=not bool(fin[k]))
+            rows.append((len(st), b.ncells[k], steps[k], tb - ta, tc - tb,
+                         same, fin[k]))
+    finally:
+        if prev_form is None:
+            os.environ.pop('SC_EULER_FORM', None)
+        else:
+            os.environ['SC_EULER_FORM'] = prev_form
+    tm = np.array(rows, dtype=float)
+    plain_march = float((tm[:, 0] * tm[:, 4]).sum())
+    plain_setup = float((tm[:, 0] * tm[:, 3]).sum())
+    # A two-term model of the sample (per-step cost plus per-cell-update
+    # cost), to report how well such a fit would have done.
+    A = np.stack([tm[:, 2], tm[:, 1] * tm[:, 2]], axis=1)
+    coef, *_ = np.linalg.lstsq(A, tm[:, 4], rcond=None)
+    resid = np.abs(A @ coef - tm[:, 4]) / tm[:, 4]
+    fit_total = float(coef[0] * steps.sum() + coef[1] * work.sum())
+    log('bench: plain sample of %d (one per stratum of %d): %.2f s '
+        'measured; whole batch serially %.1f s march + %.1f s set-up '
+        '(stratified); a two-term fit gives %.1f s, residual median %.1f%% '
+        'max %.1f%%; sample equal to the batch: %s' %
+        (len(rows), len(strata[0]), tm[:, 4].sum(), plain_march,
+         plain_setup, fit_total, 100 * np.median(resid), 100 * resid.max(),
+         bool(tm[:, 5].all())))
+    # Latency of each solve, all submitted at the march's start (set-up
+    # excluded on both sides): batched, the device time until the
+    # instance's last step ran; serially, the fitted march times of the
+    # instances before it and its own, in instance order.
+    lat = done[fin] / 1e3
+    slat = np.cumsum(coef[0] * steps + coef[1] * work)[fin]
     pct = {'p50_s': float(np.percentile(lat, 50)),
            'p99_s': float(np.percentile(lat, 99)),
            'serial_p50_s_est': float(np.percentile(slat, 50)),
@@ -398,41 +475,54 @@ def run_batched(args):
         'serially (est.)' % (pct['p50_s'], pct['p99_s'],
                              pct['serial_p50_s_est'],
                              pct['serial_p99_s_est']))
-    out = {
-        'metric': 'batched solves/s (FP64 Euler CESE, 2D triangles)',
-        'value': round(b.ninst / wall, 2),
+    res = {
+        'value': round(float(fin.sum()) / wall, 2),
         'unit': 'solves/s',
-        'n_gpus': 1,
-        'higher_is_better': True,
-        'dtype': 'f64',
-        'data': 'synthetic',
-        'config': {
-            'workload': '%d independent 2D jittered-triangle Euler solves, '
-                        'sizes log-uniform 1e3-2e5 cells, 100-1000 steps, '
-                        'Riemann boxes, CFL 0.8' % b.ninst,
-            'cells': int(b.coff[-1]),
-            'cell_updates': b.cell_updates,
-            'batched_wall_s': round(wall, 4),
-            'batched_device_ms': round(dev_ms, 2),
-            'batched_cell_updates_per_s': round(b.cell_updates / wall, 1),
-            'batched_setup_s': round(t2 - t1, 2),
-            'plain_serial_march_s_est': round(plain_march, 2),
-            'plain_serial_setup_s_est': round(plain_setup, 2),
-            'plain_sample': int(len(pick)),
-            'sample_equal_bitwise': bool(tm[:, 4].all()),
-            'latency_s': {k: round(v, 3) for k, v in pct.items()},
-        },
+        'workload': '%d independent 2D jittered-triangle Euler solves, '
+                    'sizes log-uniform 1e3-2e5 cells, 100-1000 steps, '
+                    'Riemann boxes, CFL %g' % (b.ninst, args.batch_cfl),
+        'cells': int(b.coff[-1]),
+        'cell_updates': b.cell_updates,
+        'batched_wall_s': round(wall, 4),
+        'batched_device_ms': round(dev_ms, 2),
+        'batched_cell_updates_per_s': round(b.cell_updates / wall, 1),
+        'batched_setup_s': round(t2 - t1, 2),
+        'not_finite': nbad,
+        'plain_sample': int(len(rows)),
+        'plain_sample_march_s': round(float(tm[:, 4].sum()), 3),
+        'plain_serial_march_s': round(plain_march, 2),
+        'plain_serial_setup_s': round(plain_setup, 2),
+        'plain_form': args.plain_form,
+        'plain_fit_residual_median': round(float(np.median(resid)), 4),
+        'sample_equal_bitwise': bool(tm[:, 5].all()),
+        'sample_not_finite': int((tm[:, 6] == 0).sum()),
+        'latency_s': {k: round(v, 3) for k, v in pct.items()},
+        'latency_note': 'set-up excluded on both sides; serial latencies '
+                        'from the two-term fit of the plain sample',
         'p99_s': round(pct['p99_s'], 3),
         'vs_plain': round(plain_march / wall, 3),
         'vs_plain_end_to_end': round((plain_march + plain_setup)
                                      / (wall + t2 - t1), 3),
     }
-    print(json.dumps(out), flush=True)
+    if line:
+        out = {
+            'metric': 'batched solves/s (FP64 Euler CESE, 2D triangles)',
+            'value': res['value'], 'unit': 'solves/s', 'n_gpus': 1,
+            'higher_is_better': True, 'dtype': 'f64', 'data': 'synthetic',
+            'config': {k: v for k, v in res.items()
+                       if k not in ('value', 'unit', 'p99_s', 'vs_plain',
+                                    'vs_plain_end_to_end')},
+            'p99_s': res['p99_s'], 'vs_plain': res['vs_plain'],
+            'vs_plain_end_to_end': res['vs_plain_end_to_end'],
+        }
+        print(json.dumps(out), flush=True)
+    return res
 
 
 def run_served(args):
     """The serving form: ``--serve-n`` requests of 1D/2D/3D solves in a
-    50/35/15 mix.
+    50/35/15 mix, served by the request loop's continuous batching (a
+    resident 2D and 3D arena and a 1D tube engine).
     - Sizes are log-uniform in 1e3-2e6 cells; steps are uniform in
       100-1000.
     - Arrivals follow a Poisson process at several loads, given as
@@ -457,7 +547,7 @@ def run_served(args):
     for frac in args.serve_loads:
         reqs = make_requests(args.serve_n, rate=frac * mu, seed=5)
         jobs, wall = serve(reqs, payloads,
-                           budget_bytes=int(args.serve_budget_gb * 2 ** 30))
+                           setup_budget=int(args.serve_budget_gb * 2 ** 30))
         lat = np.array([j.t_end - j.req.arrival for j in jobs])
         span = max(j.t_end for j in jobs)
         arr = np.array([r.arrival for r in reqs])
@@ -498,7 +588,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=None)
     ap.add_argument('--warmup', type=int, default=None)
-    ap.add_argument('--config', choices=('2d', '3d', '1d'), default='2d')
+    ap.add_argument('--config', choices=('2d', '3d', '1d'), default='3d')
     ap.add_argument('--size', type=int, default=1024)
     ap.add_argument('--form', default=None,
                     help='fix the multidim kernel forms instead of the '
@@ -507,7 +597,13 @@ def main():
     ap.add_argument('--size3', type=int, default=200)
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--cpu-size3', type=int, default=60)
-    ap.add_argument('--cpu-steps3', type=int, default=40)
+    ap.add_argument('--cpu-steps3', type=int, default=20)
+    ap.add_argument('--cpu-steps3-1t', type=int, default=3)
+    ap.add_argument('--no-batched', action='store_true',
+                    help='leave the batched form out of the 3D line')
+    ap.add_argument('--batch-cfl', type=float, default=0.6)
+    ap.add_argument('--plain-form', default='4:f3g',
+                    help='kernel form of the plain sample (no autotuning)')
     ap.add_argument('--copy-probe', type=int, default=1024,
                     help='MiB moved by the stream-copy probe (0: skip)')
     ap.add_argument('--no-cpu', action='store_true')
@@ -517,13 +613,14 @@ def main():
     ap.add_argument('--batched', action='store_true',
                     help='measure the batched form instead')
     ap.add_argument('--batch-n', type=int, default=4096)
-    ap.add_argument('--plain-sample', type=int, default=24)
+    ap.add_argument('--plain-sample', type=int, default=410)
     ap.add_argument('--served', action='store_true',
                     help='measure the serving form instead')
     ap.add_argument('--serve-n', type=int, default=60)
     ap.add_argument('--serve-loads', type=float, nargs='+',
                     default=[0.5, 1.0, 2.0, 4.0])
-    ap.add_argument('--serve-budget-gb', type=float, default=256.0)
+    ap.add_argument('--serve-budget-gb', type=float, default=24.0,
+                    help='device bytes of solves being set up at once')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; --gpus %d is not '
@@ -535,7 +632,7 @@ def main():
         os.environ['SC_EULER_FORM'] = args.form
     if args.graph:
         os.environ['SC_EULER_GRAPH'] = '1'
-    defaults = {'2d': (500, 20), '3d': (200, 5), '1d': (21, 3)}[args.config]
+    defaults = {'2d': (500, 20), '3d': (20, 3), '1d': (21, 3)}[args.config]
     if args.steps is None:
         args.steps = defaults[0]
     if args.warmup is None:
