@@ -540,6 +540,8 @@ def run_served(args):
         (len(base), time.perf_counter() - t0))
     sj, svc = serve_serial(base, payloads)
     mu = 1.0 / svc.mean()
+    log('bench: serial set-up %.4f s mean of the %.4f s mean service' %
+        (np.mean([j.setup_s for j in sj]), svc.mean()))
     log('bench: serial plain service %.4f s mean, capacity %.2f solves/s' %
         (svc.mean(), mu))
     loads = []
@@ -547,20 +549,30 @@ def run_served(args):
     for frac in args.serve_loads:
         reqs = make_requests(args.serve_n, rate=frac * mu, seed=5)
         jobs, wall = serve(reqs, payloads,
-                           setup_budget=int(args.serve_budget_gb * 2 ** 30))
+                           setup_budget=int(args.serve_budget_gb * 2 ** 30),
+                           setup_workers=args.serve_setup_workers)
         lat = np.array([j.t_end - j.req.arrival for j in jobs])
         span = max(j.t_end for j in jobs)
         arr = np.array([r.arrival for r in reqs])
         slat, sspan = fifo_latency(arr, svc)
+        nonfin = 0
         for a, b in zip(jobs, sj):
-            equal = equal and np.array_equal(a.result, b.result)
+            fin = bool(np.isfinite(b.result).all())
+            nonfin += not fin
+            equal = equal and np.array_equal(a.result, b.result,
+                                             equal_nan=not fin)
         row = {'load': frac, 'rate': round(frac * mu, 3),
                'solves_s': round(len(jobs) / span, 3),
                'p50_s': round(float(np.percentile(lat, 50)), 4),
                'p99_s': round(float(np.percentile(lat, 99)), 4),
                'serial_solves_s': round(len(jobs) / sspan, 3),
                'serial_p50_s': round(float(np.percentile(slat, 50)), 4),
-               'serial_p99_s': round(float(np.percentile(slat, 99)), 4)}
+               'serial_p99_s': round(float(np.percentile(slat, 99)), 4),
+               'not_finite': nonfin,
+               'setup_s_mean': round(float(np.mean([j.setup_s
+                                                   for j in jobs])), 4),
+               'admit_wait_s_mean': round(float(np.mean(
+                   [j.t_ready - j.t_start - j.setup_s for j in jobs])), 4)}
         loads.append(row)
         log('bench: load %.2f: %s' % (frac, row))
     top = loads[-1]
@@ -601,7 +613,7 @@ def main():
     ap.add_argument('--cpu-steps3-1t', type=int, default=3)
     ap.add_argument('--no-batched', action='store_true',
                     help='leave the batched form out of the 3D line')
-    ap.add_argument('--batch-cfl', type=float, default=0.6)
+    ap.add_argument('--batch-cfl', type=float, default=0.8)
     ap.add_argument('--plain-form', default='4:f3g',
                     help='kernel form of the plain sample (no autotuning)')
     ap.add_argument('--copy-probe', type=int, default=1024,
@@ -619,6 +631,7 @@ def main():
     ap.add_argument('--serve-n', type=int, default=60)
     ap.add_argument('--serve-loads', type=float, nargs='+',
                     default=[0.5, 1.0, 2.0, 4.0])
+    ap.add_argument('--serve-setup-workers', type=int, default=3)
     ap.add_argument('--serve-budget-gb', type=float, default=24.0,
                     help='device bytes of solves being set up at once')
     args = ap.parse_args()

@@ -3751,6 +3751,7 @@ struct DArena
         hipEvent_t done = nullptr;
     };
     std::vector<Slot> slots;
+    std::vector<int64_t> live;     // marching slots, in admission order
     std::deque<hipEvent_t> chunks; // events after enqueued step chunks
 
     DArena(int nd, int64_t const * type_cap, int64_t ghost_cap, int64_t nslot)
@@ -4039,6 +4040,7 @@ struct DArena
             free_slot(slot);
             throw;
         }
+        live.push_back(slot);
         return slot;
     }
 
@@ -4123,6 +4125,36 @@ struct DArena
                 SC_HIP(hipMemcpyAsync(inlet.ptr + sl.inl0 * (nd + 3), e.inlet.ptr, sizeof(double) * sl.ninl * (nd + 3),
                                       hipMemcpyDeviceToDevice, s));
             }
+            if (nd == 3)
+            {
+                // The 3D kernels read so0t: compute it for both levels of
+                // the admitted rows; the kernels that write a level then
+                // write its so0t (rows no kernel writes keep theirs).
+                std::vector<std::pair<int64_t, int64_t>> rr;
+                rr.emplace_back(sl.ghost0, e.G);
+                for (size_t k = 0; k < e.groups.size(); ++k)
+                {
+                    rr.emplace_back(Gcap + groups[sl.gidx[k]].off + sl.start[k], e.groups[k].n);
+                }
+                for (int b = 0; b < 2; ++b)
+                {
+                    for (auto const & r : rr)
+                    {
+                        if (r.second <= 0)
+                        {
+                            continue;
+                        }
+                        KP t = params();
+                        t.s0c = s0[b].ptr + r.first * q;
+                        t.s1c = s1[b].ptr + r.first * q * nd;
+                        t.gam = gam.ptr + r.first;
+                        t.st = st[b].ptr + r.first * q;
+                        t.R = r.second;
+                        k_solt<3><<<grid_for(r.second, BLK), BLK, 0, s>>>(t);
+                        SC_HIP(hipGetLastError());
+                    }
+                }
+            }
             k_arena_set_dt<<<1, 64, 0, s>>>(bhdt.ptr, bqdt.ptr, slot, e.dt * 0.5, e.dt * 0.25);
             SC_HIP(hipGetLastError());
             SC_HIP(hipHostMalloc(reinterpret_cast<void **>(&sl.host), sizeof(double) * e.N * q * (1 + nd)));
@@ -4164,28 +4196,49 @@ struct DArena
         sl = Slot{};
     }
 
+    // Each group's launch covers the blocks from the first to the last
+    // marching solve's range in it; blocks of free ranges and finished
+    // solves within that span return at once.
     void launch_groups()
     {
         KP const p = params();
-        for (auto & gr : groups)
+        for (size_t gi = 0; gi < groups.size(); ++gi)
         {
-            int64_t const hwm = gr.ra.hwm();
-            if (hwm <= 0)
+            auto & gr = groups[gi];
+            int64_t lo = gr.cap;
+            int64_t hi = 0;
+            for (int64_t slot : live)
+            {
+                Slot const & sl = slots[slot];
+                for (size_t k = 0; k < sl.gidx.size(); ++k)
+                {
+                    if (sl.gidx[k] == static_cast<int>(gi))
+                    {
+                        lo = std::min(lo, sl.start[k]);
+                        hi = std::max(hi, sl.start[k] + sl.count[k]);
+                    }
+                }
+            }
+            if (hi <= lo)
             {
                 continue;
             }
-            GV const g{gr.off, gr.cap, gr.nbr.ptr, gr.geo.ptr, hwm};
-            unsigned const nb = grid_for(hwm, BLK);
+            GV const g{gr.off + lo, gr.cap, gr.nbr.ptr + lo, gr.geo.ptr + lo, hi - lo};
+            unsigned const nb = grid_for(hi - lo, BLK);
             constexpr int MB = M_FUSED | M_BATCH;
             constexpr int MT = MB | batch_waves(CT_TRI);
+            // 3D: the fused kernel that reads so0t, stages the block's rows
+            // in LDS and writes the next level's so0t (chained so0t, the
+            // form the plain march keeps for tetrahedra and prisms).
+            constexpr int M3 = MB | M_W2 | M_STMEM | M_STAGE | M_STNEXT;
             switch (gr.type)
             {
             case CT_TRI: k_cell<CT_TRI, MT | M_GDMA><<<nb, BLK, 0, stream>>>(p, g); break;
             case CT_QUAD: k_cell<CT_QUAD, MB | batch_waves(CT_QUAD)><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_TET: k_cell<CT_TET, MB | batch_waves(CT_TET)><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_HEX: k_cell<CT_HEX, MB | batch_waves(CT_HEX)><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_PRISM: k_cell<CT_PRISM, MB | batch_waves(CT_PRISM)><<<nb, BLK, 0, stream>>>(p, g); break;
-            case CT_PYRAMID: k_cell<CT_PYRAMID, MB | batch_waves(CT_PYRAMID)><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_TET: k_cell<CT_TET, M3><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_HEX: k_cell<CT_HEX, M3><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PRISM: k_cell<CT_PRISM, M3><<<nb, BLK, 0, stream>>>(p, g); break;
+            case CT_PYRAMID: k_cell<CT_PYRAMID, M3><<<nb, BLK, 0, stream>>>(p, g); break;
             default: fail(SC_ERUNTIME, "serve arena: unsupported cell type in group");
             }
             SC_HIP(hipGetLastError());
@@ -4200,12 +4253,7 @@ struct DArena
         AllocScope const scope(stream);
         for (int64_t it = 0; it < k; ++it)
         {
-            bool any = false;
-            for (auto const & sl : slots)
-            {
-                any = any || sl.marching;
-            }
-            if (!any)
+            if (live.empty())
             {
                 break;
             }
@@ -4214,11 +4262,13 @@ struct DArena
                 cur = 1 - cur;
                 launch_groups();
             }
-            for (int64_t slot = 0; slot < max_slots; ++slot)
+            std::vector<int64_t> still;
+            for (int64_t slot : live)
             {
                 Slot & sl = slots[slot];
-                if (!sl.marching || --sl.steps_left > 0)
+                if (--sl.steps_left > 0)
                 {
+                    still.push_back(slot);
                     continue;
                 }
                 sl.marching = false;
@@ -4236,6 +4286,7 @@ struct DArena
                 }
                 SC_HIP(hipEventRecord(sl.done, stream));
             }
+            live.swap(still);
         }
         hipEvent_t ev = nullptr;
         SC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));

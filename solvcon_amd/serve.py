@@ -155,6 +155,7 @@ class Job(object):
         self.slot = None
         self.dt = None
         self.est = 0
+        self.setup_s = 0.0
 
     def setup(self):
         req = self.req
@@ -193,7 +194,9 @@ class Job(object):
     def run_blocking(self):
         """Plain serial execution of this request: set-up, its own march,
         result on the host."""
+        ts = time.perf_counter()
         self.setup()
+        self.setup_s = time.perf_counter() - ts
         s = self.solver
         if self.req.kind == '1d':
             s.march_alpha2(self.req.steps)
@@ -360,7 +363,7 @@ class Server(object):
 
     def __init__(self, cells2=12_000_000, cells3=10_000_000,
                  ghosts2=2_000_000, ghosts3=3_000_000,
-                 setup_budget=24 << 30, setup_workers=1, chunk_ms=8.0,
+                 setup_budget=24 << 30, setup_workers=3, chunk_ms=8.0,
                  max_queued=2, poll_s=2e-4):
         self.engines = {'1d': Tubes(),
                         '2d': Arena(2, arena_caps(2, cells2), ghosts2),
@@ -426,7 +429,9 @@ class Server(object):
                     del dead
                     continue
                 try:
+                    ts = time.perf_counter()
                     job.setup()
+                    job.setup_s = time.perf_counter() - ts
                 except Exception as e:  # reported by the loop
                     errors.append(e)
                     with cv:
