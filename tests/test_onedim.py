@@ -385,3 +385,37 @@ def test_gpu_api_semantics(orc):
     d.so0[:] = 0
     d.march_alpha2(1)
     assert (c.so0 == o.so0).all()
+
+
+@pytest.mark.gpu
+def test_gpu_views_held_through_containers(orc):
+    # A view of a table counts as held while any Python object references
+    # it: in a list or dict, behind a memoryview, or as a sub-view.  Held
+    # views see the state after a march, and writes through them reach the
+    # device; a view that was held and released stops being refreshed.
+    print('case: 1D views held through containers', flush=True)
+    n, dt = 201, 0.4 / 60
+    core = _gpu_sod(n, 60)
+    o = orc.sod_setup(ncoord=n, time_increment=dt)
+    box = {'list': [core.so0], 'dict': {'a': core.so1}}
+    mv = memoryview(core.cfl)
+    sub = core.so0[40:60, 1:]
+    nested = [[core.so1[::2]]]
+    core.march_alpha2(10)
+    o.march(2, 10)
+    assert (box['list'][0] == o.so0).all()
+    assert (box['dict']['a'] == o.so1).all()
+    assert (np.asarray(mv) == o.cfl).all()
+    assert (sub == o.so0[40:60, 1:]).all()
+    assert (nested[0][0] == o.so1[::2]).all()
+    # A write through the held sub-view is uploaded before the next march.
+    sub[0, 0] = 0.25
+    o.so0[40, 1] = 0.25
+    core.march_alpha2(3)
+    o.march(2, 3)
+    assert (box['list'][0] == o.so0).all()
+    assert (core.so0 == o.so0).all()
+    del box, mv, sub, nested
+    core.march_alpha2(2)
+    o.march(2, 2)
+    assert (core.so0 == o.so0).all() and (core.so1 == o.so1).all()
