@@ -310,6 +310,7 @@ def run_mesh(args, which):
             'cfl_range': [cmin, cmax],
             'kernel_forms': {CELL_NAME[t]: f for t, f in
                              (ec.kernel_forms or {}).items()},
+            'exact_passes': ec.exact_passes,
         },
         'roofline': roof,
         'cpu_baseline': cpu,

@@ -153,7 +153,10 @@ int sceuler_counts(void * h, int64_t * out);
  * counterpart either: 5, the cell count of a rank chunk of the batched march
  * (0: all ranks at once); 6, 1 when march and march_batch replay each whole
  * step from a captured hipGraph, 0 (default) to launch its kernels one by
- * one; read-only 4, 1 once the first substep has tuned the kernel forms
+ * one; read-only 7, the number of exact passes the march's fast/exact
+ * kernel pairs have run (a fast kernel met a value that is not finite or
+ * lies near the ends of the exponent range; -1 if the count cannot be
+ * read); read-only 4, 1 once the first substep has tuned the kernel forms
  * (else -1); read-only 100 + t, the form of the cells of type t: V for the
  * fused kernel in variant V (+ 100 when it reads so0t, + 1000 when it
  * prefetches its sub-face geometry into LDS), 10000 + 100 A + B for the

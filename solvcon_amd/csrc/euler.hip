@@ -94,6 +94,7 @@ enum Mode : int
     M_STAGE = 512, // stage the block's own rows in LDS for neighbour reads
     M_GDMA = 1024, // prefetch the sub-face geometry into LDS (LDS DMA)
     M_STNEXT = 4096, // also write so0t of the level being written (chained so0t)
+    M_FAST = 8192, // skip the Jacobian's structural zeros, share reciprocals; report where inexact
     M_FUSED = M_SOLN | M_BC | M_CFL | M_DSOLN,
 };
 
@@ -228,6 +229,13 @@ struct KP
     int32_t const * inst;
     double const * bhdt;
     double const * bqdt;
+    // The fast/exact pair of every plain launch: the fast kernel stores seq
+    // in *redo when one of its cells may differ from the exact arithmetic,
+    // and the exact kernel launched after it with gate set returns at once
+    // unless *redo holds that seq (then it rewrites the whole group).
+    unsigned long long * redo = nullptr;
+    unsigned long long seq = 0;
+    int gate = 0;
 };
 
 struct GV
@@ -275,6 +283,24 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     constexpr int NF = TR::NF;
     constexpr int NEQ = ND + 2;
     constexpr int NFGE = TR::NFGE;
+
+    constexpr bool FAST = (MODE & M_FAST) != 0;
+    static_assert(!FAST || (MODE & M_BATCH) == 0, "the batched kernels are exact");
+    if constexpr (!FAST)
+    {
+        if (p.gate != 0)
+        {
+            if (*p.redo != p.seq)
+            {
+                return;
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+            {
+                atomicAdd(p.redo + 1, 1ull); // exact passes run
+            }
+        }
+    }
+    FastChk chk; // FAST: whether every shortcut of this cell equals the exact arithmetic
 
     int64_t const i0 = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x;
     int64_t const i = i0 + threadIdx.x;
@@ -389,6 +415,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     int64_t const row = p.G + g.off + i;
     double const * geo = g.geo + i;
     // One sub-face geometry field of this cell, from the prefetch or HBM.
+    // The CE geometry, the neighbour lists and the rows this kernel writes
+    // stream through once per substep: their loads and stores are
+    // non-temporal, which leaves the L2 to the solution rows that the
+    // neighbours read again.
     auto gsub = [&](int fld) -> double
     {
         if constexpr (GDMA)
@@ -398,7 +428,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 return gw[(fld - G0) * 64 + lane];
             }
         }
-        return geo[fld * n];
+        return __builtin_nontemporal_load(geo + fld * n);
     };
     double hdt = p.hdt;
     double qdt = p.qdt;
@@ -409,12 +439,32 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
         qdt = p.bqdt[r];
     }
 
+    // a / d with a shared reciprocal r = recip(d) in the fast form, the
+    // division in the exact form.
+    auto recip = [&](double d) -> double
+    {
+        if constexpr (FAST)
+        {
+            chk.den(d);
+            return div_recip(d);
+        }
+        return 0.0;
+    };
+    auto quot = [&](double a, double d, double r) -> double
+    {
+        if constexpr (FAST)
+        {
+            return div_fast(a, d, r, chk);
+        }
+        return a / d;
+    };
+
     int64_t nb[NF];
     int lr[NF]; // the neighbour's row in the LDS copy, or -1
 #pragma unroll
     for (int f = 0; f < NF; ++f)
     {
-        nb[f] = g.nbr[f * n + i];
+        nb[f] = __builtin_nontemporal_load(g.nbr + f * n + i);
         lr[f] = (STAGE && nb[f] >= rbase && nb[f] < rbase + nrow) ? static_cast<int>(nb[f] - rbase) : -1;
     }
     // Neighbour solution points: CE centroid (body) or mesh centroid (ghost).
@@ -519,7 +569,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 }
             }
             double const jgam = pick(inb, lds[O_GM + l], p.gam[jg]);
-            double const bvol = geo[(LY::FV + 1 + f) * n];
+            double const bvol = __builtin_nontemporal_load(geo + (LY::FV + 1 + f) * n);
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
@@ -531,8 +581,8 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 }
                 acc[k] += fusp * bvol;
             }
-            JacS<ND> jac;
-            jac.set(jgam, js);
+            JacS<ND, FAST> jac;
+            jac.set(jgam, js, &chk);
             if constexpr ((MODE & M_STMEM) == 0)
             {
                 solt_of<ND>(jac, j1, stf);
@@ -583,7 +633,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
                         for (int m = 0; m < NEQ; ++m)
                         {
-                            dfcn += jr[m][d2] * usfc[s][m];
+                            if (!FAST || !jac_zero(ND, k, m, d2))
+                            {
+                                dfcn += jr[m][d2] * usfc[s][m];
+                            }
                         }
                         dot += dfcn * gsub(LY::FN + sf * ND + d2);
                     }
@@ -591,11 +644,17 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 }
             });
         });
-        double const cvol = geo[LY::FV * n];
+        if constexpr (FAST)
+        {
+            // The energy sum is not finite where a skipped product was NaN.
+            chk.finite(acc[NEQ - 1]);
+        }
+        double const cvol = __builtin_nontemporal_load(geo + LY::FV * n);
+        double const rcv = recip(cvol);
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
         {
-            u0[k] = acc[k] / cvol;
+            u0[k] = quot(acc[k], cvol, rcv);
         }
     }
 
@@ -654,8 +713,8 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                     {
                         // so0t of the ghost row for the level being
                         // written (calc_solt_impl, euler_march.cpp:161-191).
-                        JacS<ND> gj;
-                        gj.set(p.gam[j], o0);
+                        JacS<ND, FAST> gj;
+                        gj.set(p.gam[j], o0, &chk);
                         double gst[NEQ];
                         solt_of<ND>(gj, o1, gst);
 #pragma unroll
@@ -681,7 +740,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 u0[k] = p.s0n[(row) * NEQ + k];
             }
         }
-        double const dist = geo[LY::FD * n];
+        double const dist = __builtin_nontemporal_load(geo + LY::FD * n);
         double const ga = p.gam[row];
         double const ga1 = ga - 1.0;
         double const density = u0[0];
@@ -698,9 +757,9 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
         double const pra = (pr + fabs(pr)) / 2.0;
         double const wspd = sqrt(ga * pra / density) + sqrt(momsq) / density;
         double const co = hdt * wspd / dist;
-        p.cflo[row] = co;
+        __builtin_nontemporal_store(co, p.cflo + row);
         cflc_v = (co - 1.0) * pra / (pra + TINYJ) + 1.0;
-        p.cflc[row] = cflc_v;
+        __builtin_nontemporal_store(cflc_v, p.cflc + row);
         u0[NEQ - 1] = pra / ga1 + ke + TINYJ;
     }
     if constexpr ((MODE & (M_SOLN | M_CFL)) != 0)
@@ -708,7 +767,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
         for (int k = 0; k < NEQ; ++k)
         {
-            p.s0n[(row) * NEQ + k] = u0[k];
+            __builtin_nontemporal_store(u0[k], p.s0n + (row) * NEQ + k);
         }
     }
 
@@ -828,10 +887,11 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 cnd[d] += sub[d] * vob;
             }
         }
+        double const rvoc = recip(voc);
 #pragma unroll
         for (int d = 0; d < ND; ++d)
         {
-            cnd[d] /= voc;
+            cnd[d] = quot(cnd[d], voc, rvoc);
         }
         double idis[NF][ND];
         double udf[NF][NEQ];
@@ -887,6 +947,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
         constexpr bool ADJ_ONCE = NFGE * ND * ND <= 16;
         double adj[ADJ_ONCE ? NFGE : 1][ND][ND];
         double det[NFGE];
+        double rdet[NFGE]; // recip(det[e])
 #pragma unroll
         for (int e = 0; e < NFGE; ++e)
         {
@@ -905,6 +966,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 ge_adj<ND>(dst, adj[e]);
             }
             det[e] = ge_det<ND>(dst);
+            rdet[e] = recip(det[e]);
         }
         constexpr double ofg1 = 1.0 / NFGE;
 #pragma unroll
@@ -958,7 +1020,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                 {
-                    gk[e][d] = r[d] / det[e];
+                    gk[e][d] = quot(r[d], det[e], rdet[e]);
                     sq += gk[e][d] * gk[e][d];
                 }
                 double const wgt = 1.0 / sqrt(sq + ALMOST_ZERO);
@@ -967,10 +1029,31 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             }
             double wmax = 0.0;
             double wmin = 0.0;
+            // The weights are 1 / sqrt(sq + 1e-200) in [7e-155, 1e100] (or
+            // 0, or NaN) and their sum is no smaller than any: the quotients
+            // lie in [2^-850, 1] and take no scaling, so only their
+            // finiteness is checked.
+            double rwa = 0.0;
+            if constexpr (FAST)
+            {
+                rwa = div_recip(wacc);
+            }
 #pragma unroll
             for (int e = 0; e < NFGE; ++e)
             {
-                double const w = wk[e] / wacc - ofg1;
+                double w;
+                if constexpr (FAST)
+                {
+                    w = wk[e] * rwa;
+                    double const rem = __builtin_fma(-wacc, w, wk[e]);
+                    w = __builtin_fma(rem, rwa, w);
+                    chk.finite(w);
+                }
+                else
+                {
+                    w = wk[e] / wacc;
+                }
+                w -= ofg1;
                 wk[e] = w;
                 wmax = fmax(wmax, w);
                 wmin = fmin(wmin, w);
@@ -1013,15 +1096,22 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                     s1w[k][d] = p.s1n[(row) * (NEQ * ND) + k * ND + d];
                 }
             }
-            JacS<ND> rj;
-            rj.set(p.gam[row], u0);
+            JacS<ND, FAST> rj;
+            rj.set(p.gam[row], u0, &chk);
             double rst[NEQ];
             solt_of<ND>(rj, s1w, rst);
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {
-                p.stn[(row) * NEQ + k] = rst[k];
+                __builtin_nontemporal_store(rst[k], p.stn + (row) * NEQ + k);
             }
+        }
+    }
+    if constexpr (FAST)
+    {
+        if (!chk.ok())
+        {
+            *p.redo = p.seq;
         }
     }
 }
@@ -1932,6 +2022,10 @@ struct DEuler
     // st[b] holds so0t of the level in s0[b], s1[b].
     DevBuf<double> cen, s0[2], s1[2], st[2], stm, gam, cflo, cflc;
     int cur = 0;
+    // The fast/exact launch pair's flag and the count of exact passes that
+    // ran, and the sequence number of the last fast launch.
+    DevBuf<unsigned long long> redo;
+    unsigned long long redo_seq = 0;
 
     std::vector<EulerBoundary> bcs;
     DevBuf<int32_t> gkind, ginl, gsrc;
@@ -2046,15 +2140,40 @@ struct DEuler
         p.inst = binst.ptr;
         p.bhdt = bhdt.ptr;
         p.bqdt = bqdt.ptr;
+        p.redo = redo.ptr;
+        p.seq = 0;
+        p.gate = 0;
         return p;
     }
 
+    // Every plain launch over 3D cells is a pair: the fast form over the
+    // group, then the exact form, which returns at once unless a cell of
+    // the fast launch reported a shortcut that may not equal the exact
+    // arithmetic (a value that is not finite or lies near the ends of the
+    // exponent range); it then rewrites the whole group from the same
+    // inputs.  Each pair has its own sequence number, so a flag left by an
+    // earlier pair does not count; one replayed from a graph reuses its
+    // numbers, and a stale match only costs an exact pass.  2D groups run
+    // the exact form alone: there the fast form's checks cost the
+    // registers of a third wavefront per SIMD and gain nothing.
     template <int T, int MODE>
     void launch_cell(KP const & p, GV const & g, unsigned nb)
     {
         if constexpr (variant_built(Traits<T>::ND, MODE) && ((MODE & M_GDMA) == 0 || gdma_fits(T, MODE)))
         {
-            k_cell<T, MODE><<<nb, BLK, 0, stream>>>(p, g);
+            if constexpr (Traits<T>::ND == 3)
+            {
+                KP q = p;
+                q.seq = ++redo_seq;
+                q.gate = 0;
+                k_cell<T, MODE | M_FAST><<<nb, BLK, 0, stream>>>(q, g);
+                q.gate = 1;
+                k_cell<T, MODE><<<nb, BLK, 0, stream>>>(q, g);
+            }
+            else
+            {
+                k_cell<T, MODE><<<nb, BLK, 0, stream>>>(p, g);
+            }
         }
         else
         {
@@ -3104,6 +3223,8 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
             SC_HIP(hipMemsetAsync(b->ptr, 0, b->bytes(), s));
         }
     }
+    redo.alloc(2);
+    SC_HIP(hipMemsetAsync(redo.ptr, 0, redo.bytes(), s));
     // Boundary tables.
     gkind.alloc(G);
     ginl.alloc(G);
@@ -4579,6 +4700,16 @@ double sceuler_get_param(void * h, int which)
     case 4: return e.tuned != 0 ? 1.0 : -1.0;
     case 5: return static_cast<double>(e.chunk_cells);
     case 6: return static_cast<double>(e.use_graph);
+    case 7:
+    {
+        unsigned long long n = 0;
+        if (hipMemcpyAsync(&n, e.redo.ptr + 1, sizeof(n), hipMemcpyDeviceToHost, e.stream) != hipSuccess ||
+            hipStreamSynchronize(e.stream) != hipSuccess)
+        {
+            return -1.0;
+        }
+        return static_cast<double>(n);
+    }
     default:
         if (which >= 100 && which < 116)
         {

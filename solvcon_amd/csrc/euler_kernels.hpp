@@ -136,21 +136,116 @@ __device__ inline void static_for(F && f)
     }
 }
 
+// The fast form of the march kernels (FAST = true) takes two shortcuts
+// that leave every value unchanged where they apply, and reports where
+// they may not, so the exact form runs instead (the caller's redo pass):
+//
+// Structural zeros.  The reference evaluates every Jacobian product,
+// including the 24 of 75 entries in 3D (8 of 32 in 2D) that are the
+// constant 0 for every state (jac_zero).  For a finite x, 0 * x is +-0,
+// and adding it leaves a sum unchanged but for the sign of a zero; for an
+// infinite or NaN x it is NaN.  The fast form skips these products.  The
+// energy row has no structural zero, so when a value that one multiplies
+// is not finite the energy row's sum is not finite either (a product with
+// a non-finite value is infinite or NaN, and so is every sum it enters):
+// the fast form checks the energy sums.
+//
+// Shared denominators.  The device's IEEE division (v_div_scale,
+// v_rcp_f64, two Newton steps on the reciprocal, the product, one FMA
+// residual, v_div_fmas, v_div_fixup) scales its operands and fixes the
+// result up only near the ends of the exponent range and at zeros,
+// infinities and NaNs: it scales when d or 1 / d or a / d is subnormal,
+// when |a| < 2^-969, or when |a / d| >= 2^767.  With 2^-500 <= |d| < 2^100
+// and a = 0 or 2^-922 <= |a| < 2^261 it does neither, and its steps are:
+// refine r from v_rcp_f64(d), q = a * r, rem = a - d * q (FMA), q + rem * r
+// (FMA).  div_recip() refines r once per denominator and div_fast() takes
+// the three steps per quotient.
+//
+// FastChk keeps the checks in vector registers: the least and greatest
+// binary exponent (frexp, which gives 0 for a zero and for an infinity or
+// NaN) of the numerators and of the denominators, and a sum that turns NaN
+// when a checked value is not finite.  ok() holds when the exponents lie
+// in [-921, 261] and [-499, 100] and no checked value (a quotient, an
+// energy sum) was infinite or NaN: then every quotient equals the division
+// and no structural zero met a non-finite value.
+__host__ __device__ constexpr bool jac_zero(int nd, int k, int m, int d)
+{
+    // Row 0 is non-zero only at m = d + 1 (where it is 1) and the energy
+    // row nowhere; a momentum row k is non-zero in the density column, in
+    // its own momentum column, at d = k - 1, and in another momentum
+    // column m at d = m - 1.
+    return k == 0             ? m != d + 1
+           : k == nd + 1      ? false
+           : m == 0 || m == k ? false
+           : m == nd + 1      ? d != k - 1
+                              : d != k - 1 && d != m - 1;
+}
+
+struct FastChk
+{
+    int nmin = 0;
+    int nmax = 0;
+    int dmin = 0;
+    int dmax = 0;
+    double nan = 0.0;
+
+    __device__ void num(double v)
+    {
+        int const e = __builtin_amdgcn_frexp_exp(v);
+        nmin = e < nmin ? e : nmin;
+        nmax = e > nmax ? e : nmax;
+    }
+    __device__ void den(double v)
+    {
+        int const e = __builtin_amdgcn_frexp_exp(v);
+        dmin = e < dmin ? e : dmin;
+        dmax = e > dmax ? e : dmax;
+    }
+    __device__ void finite(double v) { nan = __builtin_fma(v, 0.0, nan); }
+    __device__ bool ok() const
+    {
+        return nmin >= -921 && nmax <= 261 && dmin >= -499 && dmax <= 100 && nan == 0.0;
+    }
+};
+
+__device__ inline double div_recip(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-d, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// a / d from r = div_recip(d); the caller has checked d.
+__device__ inline double div_fast(double a, double d, double r, FastChk & c)
+{
+    c.num(a);
+    double q = a * r;
+    double const rem = __builtin_fma(-d, q, a);
+    q = __builtin_fma(rem, r, q);
+    c.finite(q);
+    return q;
+}
+
 // The flux Jacobian of one state (EulerJacobian<ND>::update,
 // euler_march.cpp:26-158), evaluated one equation row at a time: set()
 // forms the shared scalars, row<K>() the K-th rows of the flux (f[d]) and
 // of the Jacobian (j[m][d]) with the reference's expressions.  A row is
 // only live while it is used, which keeps the register footprint of the
-// flux integral small.
-template <int ND>
+// flux integral small.  With FAST, the quotients by the density and its
+// square share their reciprocals (div_fast), checked in *chk.
+template <int ND, bool FAST = false>
 struct JacS;
 
-template <>
-struct JacS<2>
+template <bool FAST>
+struct JacS<2, FAST>
 {
     double ga, ga1, ga3, ga1h, u1, u2, u3, u4, rho2, v1, v1o2, v2, v2o2, ke2, g1ke2, vs, gretot, getot, pr;
+    double rr2; // div_recip(rho2) (FAST)
+    FastChk * chk = nullptr;
 
-    __device__ inline void set(double gamma, double const * sol)
+    __device__ inline void set(double gamma, double const * sol, FastChk * c = nullptr)
     {
         ga = gamma;
         ga1 = ga - 1;
@@ -161,15 +256,25 @@ struct JacS<2>
         u3 = sol[2];
         u4 = sol[3];
         rho2 = u1 * u1;
-        v1 = u2 / u1;
+        double r1 = 0.0;
+        if constexpr (FAST)
+        {
+            chk = c;
+            chk->den(u1);
+            chk->den(rho2);
+            r1 = div_recip(u1);
+            rr2 = div_recip(rho2);
+        }
+        auto q1 = [&](double a) { return FAST ? div_fast(a, u1, r1, *chk) : a / u1; };
+        v1 = q1(u2);
         v1o2 = v1 * v1;
-        v2 = u3 / u1;
+        v2 = q1(u3);
         v2o2 = v2 * v2;
-        ke2 = (u2 * u2 + u3 * u3) / u1;
+        ke2 = q1(u2 * u2 + u3 * u3);
         g1ke2 = ga1 * ke2;
-        vs = ke2 / u1;
+        vs = q1(ke2);
         gretot = ga * u4;
-        getot = gretot / u1;
+        getot = q1(gretot);
         pr = ga1 * u4 - ga1h * ke2;
     }
 
@@ -203,7 +308,8 @@ struct JacS<2>
         else
         {
             f[0] = (pr + u4) * v1; f[1] = (pr + u4) * v2;
-            j[0][0] = (-gretot + g1ke2) * u2 / rho2; j[0][1] = (-gretot + g1ke2) * u3 / rho2;
+            auto q2 = [&](double a) { return FAST ? div_fast(a, rho2, rr2, *chk) : a / rho2; };
+            j[0][0] = q2((-gretot + g1ke2) * u2); j[0][1] = q2((-gretot + g1ke2) * u3);
             j[1][0] = getot - ga1h * (vs + 2 * v1o2); j[1][1] = -ga1 * v1 * v2;
             j[2][0] = -ga1 * v2 * v1; j[2][1] = getot - ga1h * (vs + 2 * v2o2);
             j[3][0] = ga * v1; j[3][1] = ga * v2;
@@ -211,13 +317,15 @@ struct JacS<2>
     }
 };
 
-template <>
-struct JacS<3>
+template <bool FAST>
+struct JacS<3, FAST>
 {
     double ga, ga1, ga3, ga1h, u1, u2, u3, u4, u5, rho2, v1, v1o2, v2, v2o2, v3, v3o2, ke2, g1ke2, vs, gretot, getot,
         pr;
+    double rr2; // div_recip(rho2) (FAST)
+    FastChk * chk = nullptr;
 
-    __device__ inline void set(double gamma, double const * sol)
+    __device__ inline void set(double gamma, double const * sol, FastChk * c = nullptr)
     {
         ga = gamma;
         ga1 = ga - 1;
@@ -229,17 +337,27 @@ struct JacS<3>
         u4 = sol[3];
         u5 = sol[4];
         rho2 = u1 * u1;
-        v1 = u2 / u1;
+        double r1 = 0.0;
+        if constexpr (FAST)
+        {
+            chk = c;
+            chk->den(u1);
+            chk->den(rho2);
+            r1 = div_recip(u1);
+            rr2 = div_recip(rho2);
+        }
+        auto q1 = [&](double a) { return FAST ? div_fast(a, u1, r1, *chk) : a / u1; };
+        v1 = q1(u2);
         v1o2 = v1 * v1;
-        v2 = u3 / u1;
+        v2 = q1(u3);
         v2o2 = v2 * v2;
-        v3 = u4 / u1;
+        v3 = q1(u4);
         v3o2 = v3 * v3;
-        ke2 = (u2 * u2 + u3 * u3 + u4 * u4) / u1;
+        ke2 = q1(u2 * u2 + u3 * u3 + u4 * u4);
         g1ke2 = ga1 * ke2;
-        vs = ke2 / u1;
+        vs = q1(ke2);
         gretot = ga * u5;
-        getot = gretot / u1;
+        getot = q1(gretot);
         pr = ga1 * u5 - ga1h * ke2;
     }
 
@@ -285,9 +403,10 @@ struct JacS<3>
         else
         {
             f[0] = (pr + u5) * v1; f[1] = (pr + u5) * v2; f[2] = (pr + u5) * v3;
-            j[0][0] = (-gretot + g1ke2) * u2 / rho2;
-            j[0][1] = (-gretot + g1ke2) * u3 / rho2;
-            j[0][2] = (-gretot + g1ke2) * u4 / rho2;
+            auto q2 = [&](double a) { return FAST ? div_fast(a, rho2, rr2, *chk) : a / rho2; };
+            j[0][0] = q2((-gretot + g1ke2) * u2);
+            j[0][1] = q2((-gretot + g1ke2) * u3);
+            j[0][2] = q2((-gretot + g1ke2) * u4);
             j[1][0] = getot - ga1h * (vs + 2 * v1o2); j[1][1] = -ga1 * v1 * v2; j[1][2] = -ga1 * v1 * v3;
             j[2][0] = -ga1 * v2 * v1; j[2][1] = getot - ga1h * (vs + 2 * v2o2); j[2][2] = -ga1 * v2 * v3;
             j[3][0] = -ga1 * v3 * v1; j[3][1] = -ga1 * v3 * v2; j[3][2] = getot - ga1h * (vs + 2 * v3o2);
@@ -297,9 +416,10 @@ struct JacS<3>
 };
 
 // so0t of one row: -(J . so1c) summed over the dimension then the equation
-// (calc_solt_impl, euler_march.cpp:170-190).
-template <int ND>
-__device__ inline void solt_of(JacS<ND> const & jac, double const (&s1)[ND + 2][ND], double (&st)[ND + 2])
+// (calc_solt_impl, euler_march.cpp:170-190).  With FAST the structural
+// zeros are skipped and the energy row checked in jac.chk.
+template <int ND, bool FAST>
+__device__ inline void solt_of(JacS<ND, FAST> const & jac, double const (&s1)[ND + 2][ND], double (&st)[ND + 2])
 {
     constexpr int NEQ = ND + 2;
     static_for<0, NEQ>([&](auto kc) {
@@ -314,11 +434,18 @@ __device__ inline void solt_of(JacS<ND> const & jac, double const (&s1)[ND + 2][
 #pragma unroll
             for (int jeq = 0; jeq < NEQ; ++jeq)
             {
-                val += jr[jeq][idm] * s1[jeq][idm];
+                if (!FAST || !jac_zero(ND, ieq, jeq, idm))
+                {
+                    val += jr[jeq][idm] * s1[jeq][idm];
+                }
             }
         }
         st[ieq] = -val;
     });
+    if constexpr (FAST)
+    {
+        jac.chk->finite(st[NEQ - 1]);
+    }
 }
 
 // Row-major ND x ND helpers of the boundary conditions (euler_bc.cpp:36-81).

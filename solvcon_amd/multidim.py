@@ -202,6 +202,20 @@ class EulerCore(object):
                 var((c - 10000) // 100), var(c % 100))
         return out
 
+    @property
+    def exact_passes(self):
+        """How many exact passes the kernel pairs have run (no reference
+        counterpart).  Every kernel launch of a march or stage is a fast
+        form (the Jacobian's structural zeros skipped, quotients by one
+        denominator sharing a reciprocal) followed by the exact form, which
+        rewrites the launch's cells only when a cell of the fast form met a
+        value that is not finite or lies near the ends of the exponent
+        range.  The results are the exact form's either way."""
+        n = L.lib().sceuler_get_param(self._h, 7)
+        if n < 0:
+            raise RuntimeError('EulerCore: exact pass count unavailable')
+        return int(n)
+
     sigma0 = _param(0)
     taumin = _param(1)
     tauscale = _param(2)
