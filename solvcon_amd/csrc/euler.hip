@@ -644,11 +644,6 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 }
             });
         });
-        if constexpr (FAST)
-        {
-            // The energy sum is not finite where a skipped product was NaN.
-            chk.finite(acc[NEQ - 1]);
-        }
         double const cvol = __builtin_nontemporal_load(geo + LY::FV * n);
         double const rcv = recip(cvol);
 #pragma unroll
@@ -717,6 +712,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                         gj.set(p.gam[j], o0, &chk);
                         double gst[NEQ];
                         solt_of<ND>(gj, o1, gst);
+                        if constexpr (FAST)
+                        {
+                            chk.finite(gst[NEQ - 1]);
+                        }
 #pragma unroll
                         for (int k = 0; k < NEQ; ++k)
                         {
@@ -768,6 +767,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
         for (int k = 0; k < NEQ; ++k)
         {
             __builtin_nontemporal_store(u0[k], p.s0n + (row) * NEQ + k);
+            if constexpr (FAST && (MODE & M_SOLN) != 0)
+            {
+                chk.finite(u0[k]);
+            }
         }
     }
 
@@ -1031,8 +1034,8 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             double wmin = 0.0;
             // The weights are 1 / sqrt(sq + 1e-200) in [7e-155, 1e100] (or
             // 0, or NaN) and their sum is no smaller than any: the quotients
-            // lie in [2^-850, 1] and take no scaling, so only their
-            // finiteness is checked.
+            // lie in [2^-850, 1] and take no scaling, and one that is not
+            // finite reaches the gradients the cell writes.
             double rwa = 0.0;
             if constexpr (FAST)
             {
@@ -1047,7 +1050,6 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                     w = wk[e] * rwa;
                     double const rem = __builtin_fma(-wacc, w, wk[e]);
                     w = __builtin_fma(rem, rwa, w);
-                    chk.finite(w);
                 }
                 else
                 {
@@ -1080,6 +1082,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             for (int d = 0; d < ND; ++d)
             {
                 p.s1n[(row) * (NEQ * ND) + k * ND + d] = out[d];
+                if constexpr (FAST)
+                {
+                    chk.finite(out[d]);
+                }
             }
         }
         if constexpr ((MODE & M_STNEXT) != 0)
@@ -1100,6 +1106,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             rj.set(p.gam[row], u0, &chk);
             double rst[NEQ];
             solt_of<ND>(rj, s1w, rst);
+            if constexpr (FAST)
+            {
+                chk.finite(rst[NEQ - 1]);
+            }
 #pragma unroll
             for (int k = 0; k < NEQ; ++k)
             {

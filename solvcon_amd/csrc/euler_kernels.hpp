@@ -147,27 +147,29 @@ __device__ inline void static_for(F && f)
 // infinite or NaN x it is NaN.  The fast form skips these products.  The
 // energy row has no structural zero, so when a value that one multiplies
 // is not finite the energy row's sum is not finite either (a product with
-// a non-finite value is infinite or NaN, and so is every sum it enters):
-// the fast form checks the energy sums.
+// a non-finite value is infinite or NaN, and so is every sum it enters),
+// and neither is the energy the cell writes (or its so0t).
 //
 // Shared denominators.  The device's IEEE division (v_div_scale,
 // v_rcp_f64, two Newton steps on the reciprocal, the product, one FMA
 // residual, v_div_fmas, v_div_fixup) scales its operands and fixes the
 // result up only near the ends of the exponent range and at zeros,
 // infinities and NaNs: it scales when d or 1 / d or a / d is subnormal,
-// when |a| < 2^-969, or when |a / d| >= 2^767.  With 2^-500 <= |d| < 2^100
+// when |a| < 2^-969, or when |a / d| >= 2^767.  With 2^-500 <= |d| < 2^92
 // and a = 0 or 2^-922 <= |a| < 2^261 it does neither, and its steps are:
 // refine r from v_rcp_f64(d), q = a * r, rem = a - d * q (FMA), q + rem * r
 // (FMA).  div_recip() refines r once per denominator and div_fast() takes
-// the three steps per quotient.
+// the three steps per quotient.  A quotient that is not finite (from an
+// operand that is not, or from d = 0) reaches a value the cell writes
+// through the energy row, the gradients or the state itself.
 //
-// FastChk keeps the checks in vector registers: the least and greatest
-// binary exponent (frexp, which gives 0 for a zero and for an infinity or
-// NaN) of the numerators and of the denominators, and a sum that turns NaN
-// when a checked value is not finite.  ok() holds when the exponents lie
-// in [-921, 261] and [-499, 100] and no checked value (a quotient, an
-// energy sum) was infinite or NaN: then every quotient equals the division
-// and no structural zero met a non-finite value.
+// FastChk keeps the checks in three vector registers: the greatest biased
+// binary exponent seen (frexp, which gives 0 for a zero and for an
+// infinity or NaN; numerators biased by 921, denominators by 499 and
+// doubled, so each range maps onto [0, 1182]), and a sum that turns NaN
+// when a value the cell writes is not finite.  ok() holds when both pass:
+// then every quotient equals the division and no structural zero met a
+// non-finite value.
 __host__ __device__ constexpr bool jac_zero(int nd, int k, int m, int d)
 {
     // Row 0 is non-zero only at m = d + 1 (where it is 1) and the energy
@@ -183,29 +185,21 @@ __host__ __device__ constexpr bool jac_zero(int nd, int k, int m, int d)
 
 struct FastChk
 {
-    int nmin = 0;
-    int nmax = 0;
-    int dmin = 0;
-    int dmax = 0;
+    unsigned top = 0;
     double nan = 0.0;
 
     __device__ void num(double v)
     {
-        int const e = __builtin_amdgcn_frexp_exp(v);
-        nmin = e < nmin ? e : nmin;
-        nmax = e > nmax ? e : nmax;
+        unsigned const u = static_cast<unsigned>(__builtin_amdgcn_frexp_exp(v) + 921);
+        top = u > top ? u : top;
     }
     __device__ void den(double v)
     {
-        int const e = __builtin_amdgcn_frexp_exp(v);
-        dmin = e < dmin ? e : dmin;
-        dmax = e > dmax ? e : dmax;
+        unsigned const u = static_cast<unsigned>(2 * (__builtin_amdgcn_frexp_exp(v) + 499));
+        top = u > top ? u : top;
     }
     __device__ void finite(double v) { nan = __builtin_fma(v, 0.0, nan); }
-    __device__ bool ok() const
-    {
-        return nmin >= -921 && nmax <= 261 && dmin >= -499 && dmax <= 100 && nan == 0.0;
-    }
+    __device__ bool ok() const { return top <= 1182 && nan == 0.0; }
 };
 
 __device__ inline double div_recip(double d)
@@ -223,9 +217,7 @@ __device__ inline double div_fast(double a, double d, double r, FastChk & c)
     c.num(a);
     double q = a * r;
     double const rem = __builtin_fma(-d, q, a);
-    q = __builtin_fma(rem, r, q);
-    c.finite(q);
-    return q;
+    return __builtin_fma(rem, r, q);
 }
 
 // The flux Jacobian of one state (EulerJacobian<ND>::update,
@@ -417,7 +409,7 @@ struct JacS<3, FAST>
 
 // so0t of one row: -(J . so1c) summed over the dimension then the equation
 // (calc_solt_impl, euler_march.cpp:170-190).  With FAST the structural
-// zeros are skipped and the energy row checked in jac.chk.
+// zeros are skipped; the caller checks the energy row.
 template <int ND, bool FAST>
 __device__ inline void solt_of(JacS<ND, FAST> const & jac, double const (&s1)[ND + 2][ND], double (&st)[ND + 2])
 {
@@ -442,10 +434,6 @@ __device__ inline void solt_of(JacS<ND, FAST> const & jac, double const (&s1)[ND
         }
         st[ieq] = -val;
     });
-    if constexpr (FAST)
-    {
-        jac.chk->finite(st[NEQ - 1]);
-    }
 }
 
 // Row-major ND x ND helpers of the boundary conditions (euler_bc.cpp:36-81).
