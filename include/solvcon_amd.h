@@ -141,7 +141,7 @@ int scmesh_bc(void * h, int ibc, char * name, int nname, int32_t * facn0, int32_
  * (euler.hpp:104-117). */
 
 /* EulerCore(mesh, time_increment), euler_ce.cpp:15-29; use_sfc orders the
- * cells along a Morton curve inside each cell type (no reference
+ * cells along a Hilbert curve inside each cell type (no reference
  * counterpart; results are identical either way). */
 int sceuler_create(void * mesh, double time_increment, int use_sfc, void ** out);
 void sceuler_destroy(void * h);

@@ -3,7 +3,7 @@
 The instances (2D or 3D meshes of any size, each with its own initial
 state, time increment and step count) are joined into one disjoint-union
 ``StaticMesh`` built on the device, and one batched ``EulerCore`` marches
-all of them.  Its cells are sorted by (cell type, instance rank, Morton
+all of them.  Its cells are sorted by (cell type, instance rank, Hilbert
 code), where ranks order the instances by descending step count.  Every
 substep is one fused kernel launch per cell type over the cells of the
 instances still marching, which form a prefix of each type group.  No

@@ -5,7 +5,7 @@
  * euler_bc.cpp).
  *
  * Layout in HBM.  Body cells are grouped by cell type and, inside a group,
- * ordered along a Morton curve of their centroids; ghost rows keep the
+ * ordered along a Hilbert curve of their centroids; ghost rows keep the
  * reference order and come first.  A row is therefore 0..G-1 for ghosts and
  * G + (internal position) for body cells.  Solution tables are row-major
  * records as in the reference, table[row * width + component] (so0 and so0t
@@ -1595,8 +1595,8 @@ __global__ void k_init(int64_t R, int64_t G, int neq, int ndim, double ga, doubl
     }
 }
 
-// Morton code of the centroid for the cell ordering; the cell type sits in
-// the top four bits so that the sort also groups the types.
+// Space-filling-curve code of the centroid for the cell ordering; the cell
+// type sits in the top four bits so that the sort also groups the types.
 __device__ inline uint64_t spread2(uint64_t x)
 {
     x &= 0x3fffffffULL;
@@ -1611,7 +1611,7 @@ __device__ inline uint64_t spread2(uint64_t x)
 __device__ inline uint64_t spread3(uint64_t x)
 {
     x &= 0xfffffULL;
-    x = (x | (x << 32)) & 0x00000000ffff0000ULL ? (x | (x << 32)) & 0x001f00000000ffffULL : (x | (x << 32)) & 0x001f00000000ffffULL;
+    x = (x | (x << 32)) & 0x001f00000000ffffULL;
     x = (x | (x << 16)) & 0x001f0000ff0000ffULL;
     x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
     x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
@@ -1619,9 +1619,58 @@ __device__ inline uint64_t spread3(uint64_t x)
     return x;
 }
 
+// Hilbert index of a point of an n-dimensional grid of 2^bits cells per
+// axis, from its integer coordinates: Skilling's transform ("Programming the
+// Hilbert curve", AIP Conf. Proc. 707, 2004) turns the coordinates into the
+// transposed index in place, whose bits interleaved from the top (x[0]
+// first) are the index.  Cells consecutive along a Hilbert curve are
+// always face neighbours, so a block of consecutive cells is a compact
+// region and fewer of its cells' neighbours lie outside it than along a
+// Morton curve, which jumps.
+template <int N>
+__device__ inline uint64_t hilbert_code(uint64_t (&x)[3], int bits)
+{
+    uint64_t const m = 1ULL << (bits - 1);
+    for (uint64_t q = m; q > 1; q >>= 1)
+    {
+        uint64_t const p = q - 1;
+        for (int i = 0; i < N; ++i)
+        {
+            if (x[i] & q)
+            {
+                x[0] ^= p;
+            }
+            else
+            {
+                uint64_t const t = (x[0] ^ x[i]) & p;
+                x[0] ^= t;
+                x[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < N; ++i)
+    {
+        x[i] ^= x[i - 1];
+    }
+    uint64_t t = 0;
+    for (uint64_t q = m; q > 1; q >>= 1)
+    {
+        if (x[N - 1] & q)
+        {
+            t ^= q - 1;
+        }
+    }
+    for (int i = 0; i < N; ++i)
+    {
+        x[i] ^= t;
+    }
+    return N == 2 ? (spread2(x[0]) << 1) | spread2(x[1])
+                  : (spread3(x[0]) << 2) | (spread3(x[1]) << 1) | spread3(x[2]);
+}
+
 // Sort keys of the body cells: the cell type in the top 4 bits, then (for
-// batched solves) the instance rank in the next 20, then the Morton code of
-// the centroid.
+// batched solves) the instance rank in the next 20, then the Hilbert index
+// of the centroid.
 __global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double const * clcnd, double const * lo,
                        double const * hi, int use_sfc, int32_t const * rank, uint64_t * keys, int32_t * vals)
 {
@@ -1643,8 +1692,7 @@ __global__ void k_keys(int ndim, int32_t ncell, int32_t const * cltpn, double co
             t = t < 0 ? 0 : (t > 1 ? 1 : t);
             q[d] = static_cast<uint64_t>(t * scale);
         }
-        code = ndim == 2 ? (spread2(q[0]) | (spread2(q[1]) << 1))
-                         : (spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+        code = ndim == 2 ? hilbert_code<2>(q, bits) : hilbert_code<3>(q, bits);
     }
     if (rank != nullptr)
     {
@@ -2043,8 +2091,8 @@ struct DEuler
 
     // Batched solves of independent instances in one disjoint-union mesh:
     // ranks order the instances by descending step count, cells are sorted
-    // by (type, rank, Morton), so the cells still marching at any step are
-    // a prefix of every type group (bcum[group][rank]).
+    // by (type, rank, Hilbert index), so the cells still marching at any
+    // step are a prefix of every type group (bcum[group][rank]).
     int64_t ninst = 0;
     std::vector<int64_t> bsteps;       // by rank, non-increasing
     std::vector<int32_t> brank_of;     // instance id -> rank
@@ -3030,7 +3078,7 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
         sync();
     }
 
-    // Cell order: by type, then along the Morton curve of the centroids.
+    // Cell order: by type, then along the Hilbert curve of the centroids.
     int_of_ref.alloc(N);
     ref_of_int.alloc(N);
     if (N > 0)

@@ -1,6 +1,6 @@
 /*
  * Key-value radix sort used once per EulerCore to order the body cells
- * (cell type in the high bits, a Morton code of the cell centroid below).
+ * (cell type in the high bits, a Hilbert index of the cell centroid below).
  */
 
 #include "common.hpp"

@@ -63,7 +63,7 @@ _NONREFL, _SLIPWALL, _INLET = 0, 1, 2
 class EulerCore(object):
     """The CESE solver of the Euler equations on a StaticMesh.
 
-    ``sfc`` (not in the reference) orders the body cells along a Morton
+    ``sfc`` (not in the reference) orders the body cells along a Hilbert
     curve inside each cell-type group; results are identical either way.
     """
 
