@@ -2204,34 +2204,25 @@ struct DEuler
         return p;
     }
 
-    // Every plain launch over 3D cells is a pair: the fast form over the
-    // group, then the exact form, which returns at once unless a cell of
-    // the fast launch reported a shortcut that may not equal the exact
-    // arithmetic (a value that is not finite or lies near the ends of the
-    // exponent range); it then rewrites the whole group from the same
-    // inputs.  Each pair has its own sequence number, so a flag left by an
-    // earlier pair does not count; one replayed from a graph reuses its
-    // numbers, and a stale match only costs an exact pass.  2D groups run
-    // the exact form alone: there the fast form's checks cost the
-    // registers of a third wavefront per SIMD and gain nothing.
+    // Every plain launch is a pair: the fast form over the group, then the
+    // exact form, which returns at once unless a cell of the fast launch
+    // reported a shortcut that may not equal the exact arithmetic (a value
+    // that is not finite or lies near the ends of the exponent range); it
+    // then rewrites the whole group from the same inputs.  Each pair has
+    // its own sequence number, so a flag left by an earlier pair does not
+    // count; one replayed from a graph reuses its numbers, and a stale
+    // match only costs an exact pass.
     template <int T, int MODE>
     void launch_cell(KP const & p, GV const & g, unsigned nb)
     {
         if constexpr (variant_built(Traits<T>::ND, MODE) && ((MODE & M_GDMA) == 0 || gdma_fits(T, MODE)))
         {
-            if constexpr (Traits<T>::ND == 3)
-            {
-                KP q = p;
-                q.seq = ++redo_seq;
-                q.gate = 0;
-                k_cell<T, MODE | M_FAST><<<nb, BLK, 0, stream>>>(q, g);
-                q.gate = 1;
-                k_cell<T, MODE><<<nb, BLK, 0, stream>>>(q, g);
-            }
-            else
-            {
-                k_cell<T, MODE><<<nb, BLK, 0, stream>>>(p, g);
-            }
+            KP q = p;
+            q.seq = ++redo_seq;
+            q.gate = 0;
+            k_cell<T, MODE | M_FAST><<<nb, BLK, 0, stream>>>(q, g);
+            q.gate = 1;
+            k_cell<T, MODE><<<nb, BLK, 0, stream>>>(q, g);
         }
         else
         {
