@@ -384,11 +384,48 @@ def test_planted_non_finite_march(orc, monkeypatch, name, form):
         _plant(ge, oe, 'so1n', (c, i, (i + 1) % nd), np.nan)
     _plant(ge, oe, 'so0n', (cells[-2], 1), np.inf)
     _plant(ge, oe, 'so0n', (cells[-1], neq - 1), np.nan)
+    before = ge.exact_passes
     ge.march(1)
     oe.march(1)
     assert (~np.isfinite(oe.so0n)).sum() > 10
     assert np.isfinite(oe.so0n).sum() > oe.so0n.size // 4
     _same_with_nan(ge, oe, ('so0n', 'so1n', 'cflo', 'cflc', 'so0t'))
+    assert ge.exact_passes > before
+
+
+@pytest.mark.parametrize('name,form', [('tri16', 'f1'), ('tri16', 'f3g'),
+                                       ('mixed3d', 'f2st'),
+                                       ('mixed3d', 's1s/1')])
+def test_extreme_finite_march(orc, monkeypatch, name, form):
+    # Finite states near the ends of the exponent range: a cell of density
+    # 1e15 (its square beyond 2^92) and momenta of 1e-300 (below 2^-922).
+    # Their quotients leave the range where a shared reciprocal equals the
+    # division, the fast kernels report it and the exact ones rewrite the
+    # launch.  The state stays finite (the limiter's weights around the
+    # dense cell turn NaN in some gradients, in the oracle too) and the
+    # march equals the oracle's, NaNs included.
+    print('case: extreme finite march', name, form, flush=True)
+    monkeypatch.setenv('SC_EULER_FORM', form)
+    g, o, ge, oe = _euler_pair(orc, _defs()[name], dt=1e-4)
+    _riemann(ge, oe, o, seed=6)
+    faces = list(o.bndfcs[:, 0])
+    ge.add_nonrefl(faces)
+    oe.add_bc(orc.BC_NONREFL, faces)
+    nd, neq = ge.ndim, ge.neq
+    cells = _spread_cells(ge, 3)
+    _plant(ge, oe, 'so0n', (cells[0], 0), 1e15)
+    _plant(ge, oe, 'so0n', (cells[0], neq - 1), 1e16)
+    _plant(ge, oe, 'so0n', (cells[1], 1), 1e-300)
+    _plant(ge, oe, 'so0n', (cells[2], nd), -1e-300)
+    ge.bc_soln()
+    ge.bc_dsoln()
+    oe.call('bc_soln')
+    oe.call('bc_dsoln')
+    ge.march(2)
+    oe.march(2)
+    assert np.isfinite(oe.so0n).all()
+    _same_with_nan(ge, oe, ('so0n', 'so1n', 'cflo', 'cflc', 'so0t'))
+    assert ge.exact_passes > 0
 
 
 def _riemann(ge, oe, o, seed):
