@@ -615,8 +615,9 @@ def main():
     ap.add_argument('--no-batched', action='store_true',
                     help='leave the batched form out of the 3D line')
     ap.add_argument('--batch-cfl', type=float, default=0.8)
-    ap.add_argument('--plain-form', default='4:f3g',
-                    help='kernel form of the plain sample (no autotuning)')
+    ap.add_argument('--plain-form', default='4:f1',
+                    help='kernel form of the plain sample (no autotuning): '
+                    'the form the autotuner keeps for the 2D march')
     ap.add_argument('--copy-probe', type=int, default=1024,
                     help='MiB moved by the stream-copy probe (0: skip)')
     ap.add_argument('--no-cpu', action='store_true')
