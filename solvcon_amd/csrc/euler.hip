@@ -286,17 +286,29 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 
     constexpr bool FAST = (MODE & M_FAST) != 0;
     static_assert(!FAST || (MODE & M_BATCH) == 0, "the batched kernels are exact");
-    if constexpr (!FAST)
+    // redo[0]: the sequence number of the last fast launch that reported a
+    // cell; redo[1]: exact passes run; redo[2]: set by an exact pass, so the
+    // rest of the march runs the exact form alone (the fast kernels return
+    // at once) instead of both kernels of every pair.
+    if constexpr (FAST)
+    {
+        if (p.redo[2] != 0)
+        {
+            return;
+        }
+    }
+    else
     {
         if (p.gate != 0)
         {
-            if (*p.redo != p.seq)
+            if (*p.redo != p.seq && p.redo[2] == 0)
             {
                 return;
             }
             if (blockIdx.x == 0 && threadIdx.x == 0)
             {
                 atomicAdd(p.redo + 1, 1ull); // exact passes run
+                p.redo[2] = 1;
             }
         }
     }
@@ -2080,8 +2092,9 @@ struct DEuler
     // st[b] holds so0t of the level in s0[b], s1[b].
     DevBuf<double> cen, s0[2], s1[2], st[2], stm, gam, cflo, cflc;
     int cur = 0;
-    // The fast/exact launch pair's flag and the count of exact passes that
-    // ran, and the sequence number of the last fast launch.
+    // The fast/exact launch pairs' flag, the count of exact passes that ran
+    // and the exact-only switch of the current march (k_cell), and the
+    // sequence number of the last fast launch.
     DevBuf<unsigned long long> redo;
     unsigned long long redo_seq = 0;
 
@@ -2927,6 +2940,8 @@ struct DEuler
     {
         int64_t sub = 2 * steps;
         chain_ready = false;
+        // Each march starts with the fast kernels of its pairs enabled.
+        fill<unsigned long long>(redo.ptr + 2, 1, 0ull, stream);
         if (sub > 0 && tuned == 0)
         {
             substep_fused();
@@ -3272,7 +3287,7 @@ DEuler::DEuler(DMesh * m, double time_increment, bool sfc, int64_t nbatch, int32
             SC_HIP(hipMemsetAsync(b->ptr, 0, b->bytes(), s));
         }
     }
-    redo.alloc(2);
+    redo.alloc(3);
     SC_HIP(hipMemsetAsync(redo.ptr, 0, redo.bytes(), s));
     // Boundary tables.
     gkind.alloc(G);
