@@ -428,6 +428,49 @@ def test_extreme_finite_march(orc, monkeypatch, name, form):
     assert ge.exact_passes > 0
 
 
+@pytest.mark.parametrize('name', ['tri16', 'mixed3d'])
+@pytest.mark.parametrize('seed', [1, 2, 3])
+def test_wide_range_states_stages(orc, name, seed):
+    # Random states whose densities, pressures, velocities and gradients
+    # span sixteen orders of magnitude: the fast kernels' shared
+    # reciprocals meet quotients across most of their range (and a few
+    # outside it, where the exact pass runs).  Every stage equals the
+    # oracle's, bit for bit, NaNs included.
+    print('case: wide-range states', name, seed, flush=True)
+    g, o, ge, oe = _euler_pair(orc, _defs()[name], dt=1e-3)
+    rng = np.random.default_rng(seed)
+    nd, neq, ng = ge.ndim, ge.neq, ge.ngstcell
+    tot = ng + ge.ncell
+    rho = 10.0 ** rng.uniform(-8, 8, tot)
+    pr = 10.0 ** rng.uniform(-8, 8, tot)
+    vel = rng.standard_normal((tot, nd)) * 10.0 ** rng.uniform(-6, 3, (tot, 1))
+    s0 = np.empty((tot, neq))
+    s0[:, 0] = rho
+    s0[:, 1:1 + nd] = rho[:, None] * vel
+    s0[:, -1] = pr / (GAMMA - 1.0) + 0.5 * rho * (vel ** 2).sum(1)
+    s1 = s0[:, :, None] * rng.standard_normal((tot, neq, nd)) * 10.0 ** rng.uniform(-4, 1, (tot, 1, 1))
+    st = s0 * rng.standard_normal((tot, neq)) * 1e-2
+    for arr, val in (('so0c', s0), ('so0n', s0), ('so1c', s1), ('so1n', s1), ('so0t', st)):
+        getattr(ge, arr).ndarray[...] = val
+        getattr(oe, arr)[...] = val
+    ge.gamma.ndarray[...] = GAMMA
+    oe.gamma[...] = GAMMA
+    cf = rng.uniform(0.2, 0.9, tot)
+    ge.cflc.ndarray[...] = cf
+    oe.cflc[...] = cf
+    faces = list(o.bndfcs[:, 0])
+    ge.add_nonrefl(faces[0::2])
+    oe.add_bc(orc.BC_NONREFL, faces[0::2])
+    ge.add_slipwall(faces[1::2])
+    oe.add_bc(orc.BC_SLIPWALL, faces[1::2])
+    for stage in ('calc_soln', 'calc_solt', 'bc_soln', 'calc_cfl',
+                  'calc_dsoln', 'bc_dsoln'):
+        getattr(ge, stage)()
+        oe.call(stage)
+        _same_with_nan(ge, oe, ('so0n', 'so0t', 'so1n', 'cflo', 'cflc'))
+    assert np.isfinite(oe.so0n).mean() > 0.5
+
+
 def _riemann(ge, oe, o, seed):
     from solvcon_amd.gen import riemann_box, quadrant_state
     rho, p = riemann_box(seed)
