@@ -838,6 +838,63 @@ def test_graph_replay_equals_eager():
 
 
 @pytest.mark.gpu
+def test_graph_replay_with_exact_passes():
+    # A captured march replays its fast and exact pairs with the sequence
+    # numbers of the capture.  From a state the fast kernels report (a
+    # density of 1e15 and momenta of 1e-300 in single cells) the replays run
+    # the exact forms; a later replay from a state in range may find a
+    # number stored by an earlier replay, which costs an exact pass and
+    # changes no value.  Every replay equals the eager march.
+    print('case: graph replay with exact passes', flush=True)
+    import solvcon_amd as sc
+    g = pair.gpu_mesh(_defs()['mixed3d'])
+    faces = g.bndfcs.ndarray[:, 0].copy()
+
+    def state(e, extreme):
+        u = e.so0n.ndarray
+        u[...] = 0.0
+        u[:, 0] = 1.0 + np.linspace(0.0, 0.4, u.shape[0])
+        u[:, 1] = 0.1 * u[:, 0]
+        u[:, -1] = 2.5 + 0.5 * 0.1 ** 2 * u[:, 0]
+        if extreme:
+            cells = _spread_cells(e, 3)
+            u[cells[0], 0] = 1e15
+            u[cells[0], -1] = 1e16
+            u[cells[1], 1] = 1e-300
+            u[cells[2], e.ndim] = -1e-300
+        e.so0c.ndarray[...] = u
+        del u
+        e.so1n.ndarray[...] = 0.0
+        e.so1c.ndarray[...] = 0.0
+        e.bc_soln()
+        e.bc_dsoln()
+
+    def fresh(graph):
+        e = sc.EulerCore(mesh=g, time_increment=1e-4)
+        e.use_graph = graph
+        e.init_solution(GAMMA, 1.0, [0.0, 0.0, 0.0], 1.0)
+        e.add_nonrefl(faces[0::2])
+        e.add_slipwall(faces[1::2])
+        return e
+    a = fresh(True)
+    b = fresh(False)
+    for k, (extreme, steps) in enumerate(((True, 2), (True, 2),
+                                          (False, 2), (False, 3),
+                                          (True, 2), (False, 2))):
+        state(a, extreme)
+        state(b, extreme)
+        before = a.exact_passes
+        a.march(steps)
+        b.march(steps)
+        if extreme:
+            assert a.exact_passes > before, k
+        for t in ('so0n', 'so1n', 'so0c', 'so1c', 'so0t', 'cflo', 'cflc'):
+            x, y = getattr(a, t).ndarray, getattr(b, t).ndarray
+            assert np.array_equal(x, y, equal_nan=True), (k, t)
+    assert b.exact_passes > 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize('form', ['6:f2st,7:s1s/2s,5:s1s/1,8:s1s/2', 'f2'])
 def test_chained_so0t_across_calls(orc, monkeypatch, form):
     # With forms that read so0t, each level's kernels write the next
