@@ -32,3 +32,16 @@ def test_device_count_without_gpu_is_safe():
     # sc_device_count never fails; the solvers refuse to start without one.
     from solvcon_amd import _lib
     assert _lib.lib().sc_device_count() >= 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # bench.py measures one GPU; asked for more it says so and exits 2
+    # before it touches a device.
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'),
+                        '--gpus', '2'], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert 'one GPU' in r.stderr + r.stdout
+    assert r.stdout.strip() == ''
