@@ -314,6 +314,18 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     }
     FastChk chk; // FAST: whether every shortcut of this cell equals the exact arithmetic
 
+    // Wave priority.  At one to three waves per SIMD a wave that waits to
+    // issue loads behind a wave busy with FP64 arithmetic leaves the memory
+    // pipeline idle.  A wave therefore runs at a raised priority while it
+    // issues the loads that start its work (the block's staged rows, the
+    // neighbour lists and, in 3D, the first face's neighbour row) and the
+    // gradient's neighbour gathers, and at the base priority for the
+    // arithmetic in between.  Only the issue order changes, no value.
+    //   prio_load(): raise; prio_math(): lower.
+    auto prio_load = [] { __builtin_amdgcn_s_setprio(2); };
+    auto prio_math = [] { __builtin_amdgcn_s_setprio(0); };
+    prio_load();
+
     int64_t const i0 = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x;
     int64_t const i = i0 + threadIdx.x;
     // Batched launches may cover cells of no marching instance: the serving
@@ -479,6 +491,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
         nb[f] = __builtin_nontemporal_load(g.nbr + f * n + i);
         lr[f] = (STAGE && nb[f] >= rbase && nb[f] < rbase + nrow) ? static_cast<int>(nb[f] - rbase) : -1;
     }
+    if constexpr (ND == 2)
+    {
+        prio_math();
+    }
     // Neighbour solution points: CE centroid (body) or mesh centroid (ghost).
     // A neighbour value is read from both places with no branch: from LDS
     // at its staged row (row 0 when not staged) and from global memory at
@@ -592,6 +608,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                     fusp += (geo[(LY::FB + f * ND + d) * n] - jcef[d]) * j1[k][d];
                 }
                 acc[k] += fusp * bvol;
+            }
+            if constexpr (ND == 3 && f == 0)
+            {
+                prio_math();
             }
             JacS<ND, FAST> jac;
             jac.set(jgam, js, &chk);
@@ -788,6 +808,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
 
     if constexpr ((MODE & M_DSOLN) != 0)
     {
+        if constexpr (ND == 3)
+        {
+            prio_load();
+        }
         // calc_dsoln (euler_gradient.cpp:23-130) with GradientElement
         // (GradientElement.cpp:137-213).
         if constexpr ((MODE & M_CFL) == 0)
@@ -910,6 +934,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
         }
         double idis[NF][ND];
         double udf[NF][NEQ];
+        if constexpr (ND == 2)
+        {
+            prio_load();
+        }
 #pragma unroll
         for (int f = 0; f < NF; ++f)
         {
@@ -954,6 +982,7 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 udf[f][k] = val;
             }
         }
+        prio_math();
         // Displacement-matrix adjugates and determinants of every FGE
         // depend only on geometry.  In 2D all adjugates stay in registers
         // across the equations; in 3D (8 x 9 values for a hexahedron) each
