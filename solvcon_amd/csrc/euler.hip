@@ -319,8 +319,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     // pipeline idle.  A wave therefore runs at a raised priority while it
     // issues the loads that start its work (the block's staged rows, the
     // neighbour lists and, in 3D, the first face's neighbour row) and the
-    // gradient's neighbour gathers, and at the base priority for the
-    // arithmetic in between.  Only the issue order changes, no value.
+    // loads of the gradient (in 3D the whole section up to the displacement
+    // matrices, in 2D its neighbour gathers), and at the base priority for
+    // the arithmetic in between.  The placement is the fastest of those
+    // measured per dimension.  Only the issue order changes, no value.
     //   prio_load(): raise; prio_math(): lower.
     auto prio_load = [] { __builtin_amdgcn_s_setprio(2); };
     auto prio_math = [] { __builtin_amdgcn_s_setprio(0); };
