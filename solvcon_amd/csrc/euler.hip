@@ -319,10 +319,11 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
     // pipeline idle.  A wave therefore runs at a raised priority while it
     // issues the loads that start its work (the block's staged rows, the
     // neighbour lists and, in 3D, the first face's neighbour row) and the
-    // loads of the gradient (in 3D the whole section up to the displacement
-    // matrices, in 2D its neighbour gathers), and at the base priority for
-    // the arithmetic in between.  The placement is the fastest of those
-    // measured per dimension.  Only the issue order changes, no value.
+    // loads of the gradient (in 3D the whole section through the
+    // displacement matrices, in 2D its neighbour gathers), and at the base
+    // priority for the arithmetic in between.  The placement is the fastest
+    // of those measured per dimension.  Only the issue order changes, no
+    // value.
     //   prio_load(): raise; prio_math(): lower.
     auto prio_load = [] { __builtin_amdgcn_s_setprio(2); };
     auto prio_math = [] { __builtin_amdgcn_s_setprio(0); };
@@ -984,7 +985,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
                 udf[f][k] = val;
             }
         }
-        prio_math();
+        if constexpr (ND == 2)
+        {
+            prio_math();
+        }
         // Displacement-matrix adjugates and determinants of every FGE
         // depend only on geometry.  In 2D all adjugates stay in registers
         // across the equations; in 3D (8 x 9 values for a hexahedron) each
@@ -1013,6 +1017,10 @@ __global__ void __launch_bounds__(BLK, waves_of(MODE)) k_cell(KP const p, GV con
             }
             det[e] = ge_det<ND>(dst);
             rdet[e] = recip(det[e]);
+        }
+        if constexpr (ND == 3)
+        {
+            prio_math();
         }
         constexpr double ofg1 = 1.0 / NFGE;
 #pragma unroll
